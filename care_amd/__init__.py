@@ -1,0 +1,326 @@
+"""care_amd: MI355X-native CARE read-correction engine.
+
+Python bindings (ctypes) over the C-ABI declared in include/care_amd.h
+(engine: read store in HBM, device minhash tables, fused correction kernel)
+and include/care_host.h (read packing, the care-gpu command line).
+
+The native libraries are built in-tree by care_amd.build (or
+__graft_entry__.build()).  Nothing here falls back to a CPU path: if the HIP
+engine is missing, loading fails with an error.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD_DIR = os.path.join(HERE, "_build")
+ENGINE_LIB = os.path.join(BUILD_DIR, "libcare_amd.so")
+HOST_LIB = os.path.join(BUILD_DIR, "libcare_host.so")
+CARE_GPU = os.path.join(BUILD_DIR, "care-gpu")
+READSIM = os.path.join(BUILD_DIR, "readsim")
+
+CARE_OK = 0
+
+
+class CareOptions(ctypes.Structure):
+    """care_options (include/care_amd.h); defaults of include/options.hpp:36-98."""
+
+    _fields_ = [
+        ("kmer_length", ctypes.c_int),
+        ("num_hash_maps", ctypes.c_int),
+        ("coverage", ctypes.c_float),
+        ("paired_end", ctypes.c_int),
+        ("use_quality", ctypes.c_int),
+        ("exclude_ambiguous", ctypes.c_int),
+        ("correct_candidates", ctypes.c_int),
+        ("new_columns_to_correct", ctypes.c_int),
+        ("min_overlap", ctypes.c_int),
+        ("max_mismatch_ratio", ctypes.c_float),
+        ("min_overlap_ratio", ctypes.c_float),
+        ("error_factor", ctypes.c_float),
+        ("coverage_factor", ctypes.c_float),
+        ("paired_filter_threshold", ctypes.c_float),
+        ("correction_type", ctypes.c_int),
+        ("correction_type_cands", ctypes.c_int),
+        ("threshold_anchor", ctypes.c_float),
+        ("threshold_cands", ctypes.c_float),
+    ]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [
+        ("anchor_status", ctypes.POINTER(ctypes.c_uint8)),
+        ("anchor_sequences", ctypes.POINTER(ctypes.c_char)),
+        ("sequence_pitch", ctypes.c_int),
+        ("candidate_read_ids", ctypes.POINTER(ctypes.c_uint32)),
+        ("candidate_sequences", ctypes.POINTER(ctypes.c_char)),
+        ("candidate_capacity", ctypes.c_int),
+        ("num_candidates", ctypes.c_int),
+    ]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("query_ms", ctypes.c_double),
+        ("correct_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("raw_hits", ctypes.c_uint64),
+        ("candidates", ctypes.c_uint64),
+    ]
+
+
+# exported C symbols and their signatures
+ENGINE_SYMBOLS = {
+    "care_options_default": (None, [ctypes.POINTER(CareOptions)]),
+    "care_last_error": (ctypes.c_char_p, []),
+    "care_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CareOptions), ctypes.POINTER(ctypes.c_void_p)]),
+    "care_context_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "care_readstore_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_int, ctypes.c_int]),
+    "care_readstore_gather": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_int, ctypes.c_void_p]),
+    "care_minhash_build": (ctypes.c_int, [ctypes.c_void_p]),
+    "care_minhash_signatures": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]),
+    "care_minhash_query": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_uint64]),
+    "care_correct_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                          ctypes.POINTER(BatchResult)]),
+    "care_set_launch_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "care_forest_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint32]),
+    "care_last_batch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchStats)]),
+}
+
+HOST_SYMBOLS = {
+    "care_host_main": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
+    "care_host_last_error": (ctypes.c_char_p, []),
+    "care_readset_load": (ctypes.c_void_p, [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int]),
+    "care_readset_free": (None, [ctypes.c_void_p]),
+    "care_readset_size": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "care_readset_max_length": (ctypes.c_int, [ctypes.c_void_p]),
+    "care_readset_seq_pitch": (ctypes.c_int, [ctypes.c_void_p]),
+    "care_readset_qual_pitch": (ctypes.c_int, [ctypes.c_void_p]),
+    "care_readset_seqs": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "care_readset_lengths": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "care_readset_ambiguous": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "care_readset_quals": (ctypes.c_void_p, [ctypes.c_void_p]),
+}
+
+_engine = None
+_host = None
+
+
+def _bind(lib, table):
+    for name, (res, args) in table.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def engine():
+    """The HIP engine library; raises if it has not been built."""
+    global _engine
+    if _engine is None:
+        if not os.path.exists(ENGINE_LIB):
+            raise ImportError("care_amd HIP engine not built (%s missing); run __graft_entry__.build()" % ENGINE_LIB)
+        _engine = _bind(ctypes.CDLL(ENGINE_LIB, mode=ctypes.RTLD_GLOBAL), ENGINE_SYMBOLS)
+    return _engine
+
+
+def host():
+    global _host
+    if _host is None:
+        engine()
+        if not os.path.exists(HOST_LIB):
+            raise ImportError("care_amd host library not built (%s missing)" % HOST_LIB)
+        _host = _bind(ctypes.CDLL(HOST_LIB), HOST_SYMBOLS)
+    return _host
+
+
+class CareError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != CARE_OK:
+        raise CareError("care_amd error %d: %s" % (rc, engine().care_last_error().decode()))
+
+
+def default_options(**kw):
+    opt = CareOptions()
+    engine().care_options_default(ctypes.byref(opt))
+    for k, v in kw.items():
+        if not hasattr(opt, k):
+            raise KeyError(k)
+        setattr(opt, k, v)
+    return opt
+
+
+class ReadSet:
+    """Packed reads (2-bit, N rule, binned qualities) parsed by the C++ host."""
+
+    def __init__(self, files, paired_end=False, qual_bits=0):
+        h = host()
+        arr = (ctypes.c_char_p * len(files))(*[f.encode() for f in files])
+        self._p = h.care_readset_load(arr, len(files), int(paired_end), int(qual_bits))
+        if not self._p:
+            raise CareError("read loading failed: %s" % h.care_host_last_error().decode())
+        self.n = int(h.care_readset_size(self._p))
+        self.max_length = int(h.care_readset_max_length(self._p))
+        self.pitch = int(h.care_readset_seq_pitch(self._p))
+        self.qpitch = int(h.care_readset_qual_pitch(self._p))
+        self.qual_bits = int(qual_bits) if self.qpitch else 0
+
+    def _array(self, fn, dtype, count):
+        ptr = fn(self._p)
+        if not ptr or count == 0:
+            return np.zeros(0, dtype=dtype)
+        buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+        return np.frombuffer(buf, dtype=dtype).copy()
+
+    @property
+    def seqs(self):
+        return self._array(host().care_readset_seqs, np.uint32, self.n * self.pitch).reshape(self.n, self.pitch)
+
+    @property
+    def lengths(self):
+        return self._array(host().care_readset_lengths, np.int32, self.n)
+
+    @property
+    def ambiguous(self):
+        return self._array(host().care_readset_ambiguous, np.uint8, self.n)
+
+    @property
+    def quals(self):
+        return self._array(host().care_readset_quals, np.uint32, self.n * self.qpitch).reshape(self.n, max(self.qpitch, 1))
+
+    def close(self):
+        if self._p:
+            host().care_readset_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One device context of the HIP engine."""
+
+    def __init__(self, options, device=0):
+        self.lib = engine()
+        self.options = options
+        ctx = ctypes.c_void_p()
+        _check(self.lib.care_context_create(device, ctypes.byref(options), ctypes.byref(ctx)))
+        self.ctx = ctx
+        self.n = 0
+        self.max_length = 0
+
+    def close(self):
+        if self.ctx:
+            self.lib.care_context_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, seqs, lengths, ambiguous=None, quals=None, qual_bits=8, max_length=None):
+        seqs = np.ascontiguousarray(seqs, dtype=np.uint32)
+        lengths = np.ascontiguousarray(lengths, dtype=np.int32)
+        n, pitch = seqs.shape
+        amb = None if ambiguous is None else np.ascontiguousarray(ambiguous, dtype=np.uint8)
+        q = None if quals is None else np.ascontiguousarray(quals, dtype=np.uint32)
+        ml = int(max_length if max_length is not None else lengths.max())
+        _check(self.lib.care_readstore_upload(
+            self.ctx, n, ml, seqs.ctypes.data, pitch, lengths.ctypes.data,
+            None if amb is None else amb.ctypes.data, None if q is None else q.ctypes.data,
+            0 if q is None else q.shape[1], qual_bits))
+        self.n = n
+        self.max_length = ml
+
+    def upload_readset(self, rs):
+        quals = rs.quals if (self.options.use_quality and rs.qpitch) else None
+        self.upload(rs.seqs, rs.lengths, rs.ambiguous, quals, rs.qual_bits or 8, rs.max_length)
+
+    def build(self):
+        _check(self.lib.care_minhash_build(self.ctx))
+
+    def set_launch_config(self, batch=0, threads=0):
+        _check(self.lib.care_set_launch_config(self.ctx, batch, threads))
+
+    def signatures(self, first, n):
+        out = np.zeros((n, self.options.num_hash_maps), dtype=np.uint64)
+        _check(self.lib.care_minhash_signatures(self.ctx, first, n, out.ctypes.data))
+        return out
+
+    def query(self, first, n, capacity=None):
+        cap = capacity if capacity is not None else max(1, n) * 64
+        while True:
+            ids = np.zeros(cap, dtype=np.uint32)
+            offs = np.zeros(n + 1, dtype=np.int32)
+            rc = self.lib.care_minhash_query(self.ctx, first, n, ids.ctypes.data, offs.ctypes.data, cap)
+            if rc == 4 and capacity is None:
+                cap *= 4
+                continue
+            _check(rc)
+            return ids[: offs[n]], offs
+
+    def correct(self, first, n, candidate_capacity=None):
+        pitch = (self.max_length + 15) // 16 * 16
+        status = np.zeros(n, dtype=np.uint8)
+        seqs = np.zeros((n, pitch), dtype=np.uint8)
+        ccap = candidate_capacity if candidate_capacity is not None else (n * 64 if self.options.correct_candidates else 0)
+        cids = np.zeros(max(ccap, 1), dtype=np.uint32)
+        cseqs = np.zeros((max(ccap, 1), pitch), dtype=np.uint8)
+        r = BatchResult()
+        r.anchor_status = status.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        r.anchor_sequences = seqs.ctypes.data_as(ctypes.POINTER(ctypes.c_char))
+        r.sequence_pitch = pitch
+        r.candidate_read_ids = cids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        r.candidate_sequences = cseqs.ctypes.data_as(ctypes.POINTER(ctypes.c_char))
+        r.candidate_capacity = ccap
+        _check(self.lib.care_correct_batch(self.ctx, first, n, ctypes.byref(r)))
+        nc = r.num_candidates
+        return status, seqs, cids[:nc], cseqs[:nc]
+
+    def stats(self):
+        s = BatchStats()
+        _check(self.lib.care_last_batch_stats(self.ctx, ctypes.byref(s)))
+        return s
+
+
+def run_care_gpu(args, **kw):
+    """Runs the care-gpu command line (a child process)."""
+    if not os.path.exists(CARE_GPU):
+        raise ImportError("care-gpu not built (%s missing)" % CARE_GPU)
+    return subprocess.run([CARE_GPU] + list(args), **kw)
+
+
+def simulate(profile, seed, out, truth, genome=None, coverage=None, length=None):
+    """Seeded synthetic reads (care_amd/sim/readsim.cpp)."""
+    cmd = [READSIM, "gen", "--profile", profile, "--seed", str(seed), "--out", out, "--truth", truth]
+    if genome is not None:
+        cmd += ["--genome", str(int(genome))]
+    if coverage is not None:
+        cmd += ["--coverage", str(coverage)]
+    if length is not None:
+        cmd += ["--length", str(int(length))]
+    subprocess.run(cmd, check=True)
+
+
+def evaluate(orig, corrected, truth):
+    """Base-level TP/FP/FN/TN (evaluationtool/arteval.cpp:183-227 definitions)."""
+    import json
+    p = subprocess.run([READSIM, "eval", "--orig", orig, "--corr", corrected, "--truth", truth],
+                       check=True, stdout=subprocess.PIPE, text=True)
+    return json.loads(p.stdout)

@@ -1,0 +1,646 @@
+// C-ABI of the engine: contexts, read store upload, table construction,
+// queries and batch correction.  Declared in include/care_amd.h.
+
+#include "../../include/care_amd.h"
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace care_engine {
+// launchers of minhash.hip
+void launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s);
+void launch_fill_u32(u32* p, u64 n, u32 v, hipStream_t s);
+void launch_count_keys(const u64* sigs, u64 n, int h, int map, u64* tkeys, u32* tcount, u64 tmask, int* overflow,
+                       hipStream_t s);
+void launch_assign_offsets(const u64* tkeys, const u32* tcount, u64 tsize, int cap, u32* toff,
+                           unsigned long long* counters, hipStream_t s);
+void launch_insert_final(const u64* tkeys, const u32* tcount, const u32* toff, u64 tsize, u64* fkeys, uint2* foffcnt,
+                         u64 fmask, int* overflow, hipStream_t s);
+void launch_scatter_values(const u64* sigs, u64 n, int h, int map, const u64* tkeys, const u32* toff, u64 tmask,
+                           u32* tfill, u32* values, hipStream_t s);
+void launch_sort_buckets(const u32* toff, const u32* tcount, u64 tsize, u32* values, hipStream_t s);
+void launch_reset_counters(int* queue, int* candOut, hipStream_t s);
+}  // namespace care_engine
+
+using namespace care_engine;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_lastError = buf;
+    return code;
+}
+
+struct HipError {
+    hipError_t err;
+    const char* what;
+};
+
+#define HIP_TRY(expr)                                   \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) throw HipError{e_, #expr}; \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        HIP_TRY(hipMalloc(&p, sizeof(T) * count));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+template <class T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count, hipHostMallocDefault));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~PinnedBuf() { release(); }
+};
+
+struct MapStore {
+    DevBuf<u64> keys;
+    DevBuf<uint2> offcnt;
+    DevBuf<u32> values;
+    u64 mask = 0;
+};
+
+int resultsPerMapThreshold(float coverage) {
+    // include/minhasherlimit.hpp:10-15
+    int r = int(coverage * 2.5f);
+    r = std::min(r, 65535);
+    return std::max(10, r);
+}
+
+}  // namespace
+
+struct care_context {
+    int device = 0;
+    care_options opt{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4]{};
+    int numCUs = 0;
+
+    // read store
+    DevBuf<u32> seqs;
+    DevBuf<int> lens;
+    DevBuf<u8> ambig;
+    DevBuf<u32> quals;
+    DevReadStore rs{};
+    bool hasReads = false;
+
+    // tables
+    std::vector<std::unique_ptr<MapStore>> maps;
+    DevBuf<DevMap> devMaps;
+    DevTables tables{};
+    bool hasTables = false;
+    int cap = 0;
+
+    // batch buffers
+    int batchCap = 0;
+    int maxCand = 0;
+    int blockThreads = 256;
+    int requestedBatch = 0;
+    DevBuf<u32> candIds;
+    DevBuf<int> candCount;
+    DevBuf<CandMeta> metaA, metaB;
+    DevBuf<u8> outStatus;
+    DevBuf<char> outSeq;
+    int outPitch = 0;
+    DevBuf<u32> candOutIds;
+    DevBuf<char> candOutSeq;
+    int candOutCap = 0;
+    DevBuf<int> counters;  // [0] queue head, [1] candidate output count
+    DevBuf<unsigned long long> rawHits;
+    DevBuf<float> qweights;
+    PinnedBuf<u8> hStatus;
+    PinnedBuf<char> hSeq;
+    PinnedBuf<u32> hCandIds;
+    PinnedBuf<char> hCandSeq;
+    PinnedBuf<int> hInts;
+    PinnedBuf<unsigned long long> hU64;
+
+    care_batch_stats stats{};
+};
+
+namespace {
+
+// Batch buffers sized for `batch` anchors.
+void ensure_batch(care_context* ctx, int batch) {
+    if (batch <= ctx->batchCap) return;
+    const size_t mc = size_t(ctx->maxCand);
+    ctx->candIds.alloc(size_t(batch) * mc);
+    ctx->candCount.alloc(batch);
+    ctx->metaA.alloc(size_t(batch) * mc);
+    ctx->metaB.alloc(size_t(batch) * mc);
+    ctx->outStatus.alloc(batch);
+    ctx->outPitch = (ctx->rs.maxLen + 15) / 16 * 16;
+    ctx->outSeq.alloc(size_t(batch) * ctx->outPitch);
+    ctx->hStatus.alloc(batch);
+    ctx->hSeq.alloc(size_t(batch) * ctx->outPitch);
+    if (ctx->opt.correct_candidates) {
+        // every HQ anchor corrects at most its candidates within the window;
+        // a generous average bound, checked per batch
+        ctx->candOutCap = int(std::min<size_t>(size_t(batch) * 64, size_t(1) << 26));
+        ctx->candOutIds.alloc(ctx->candOutCap);
+        ctx->candOutSeq.alloc(size_t(ctx->candOutCap) * ctx->outPitch);
+        ctx->hCandIds.alloc(ctx->candOutCap);
+        ctx->hCandSeq.alloc(size_t(ctx->candOutCap) * ctx->outPitch);
+    }
+    ctx->batchCap = batch;
+}
+
+int default_batch(care_context* ctx) {
+    if (ctx->requestedBatch > 0) return ctx->requestedBatch;
+    // bound scratch to ~2 GiB: candidate ids + two metadata copies per slot
+    const size_t perAnchor = size_t(ctx->maxCand) * (sizeof(u32) + 2 * sizeof(CandMeta)) + ctx->rs.maxLen + 64;
+    size_t b = (size_t(2) << 30) / perAnchor;
+    b = std::max<size_t>(256, std::min<size_t>(b, 65536));
+    return int(b & ~size_t(1));
+}
+
+}  // namespace
+
+namespace care_engine {
+__global__ void k_reset_counters(int* queue, int* candOut) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        queue[0] = 0;
+        if (candOut) candOut[0] = 0;
+    }
+}
+void launch_reset_counters(int* queue, int* candOut, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, queue, candOut);
+}
+}  // namespace care_engine
+
+extern "C" {
+
+void care_options_default(care_options* o) {
+    if (!o) return;
+    o->kmer_length = 20;
+    o->num_hash_maps = 48;
+    o->coverage = 1.0f;
+    o->paired_end = 0;
+    o->use_quality = 0;
+    o->exclude_ambiguous = 0;
+    o->correct_candidates = 0;
+    o->new_columns_to_correct = 15;
+    o->min_overlap = 30;
+    o->max_mismatch_ratio = 0.2f;
+    o->min_overlap_ratio = 0.30f;
+    o->error_factor = 0.06f;
+    o->coverage_factor = 0.6f;
+    o->paired_filter_threshold = 0.06f;
+    o->correction_type = 0;
+    o->correction_type_cands = 0;
+    o->threshold_anchor = 0.5f;
+    o->threshold_cands = 0.5f;
+}
+
+const char* care_last_error(void) { return g_lastError.c_str(); }
+
+int care_context_create(int device, const care_options* opt, care_context** out) {
+    if (!out || !opt) return fail(CARE_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (opt->kmer_length < 1 || opt->kmer_length > 31)
+        return fail(CARE_ERR_ARGUMENT, "kmer_length %d outside 1..31", opt->kmer_length);
+    if (opt->num_hash_maps < 1 || opt->num_hash_maps > 64)
+        return fail(CARE_ERR_ARGUMENT, "num_hash_maps %d outside 1..64", opt->num_hash_maps);
+    if (opt->correction_type != 0 || opt->correction_type_cands != 0)
+        return fail(CARE_ERR_ARGUMENT, "only correction type 0 (classic) is available in this build");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CARE_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= count) return fail(CARE_ERR_ARGUMENT, "device %d outside 0..%d", device, count - 1);
+    auto* ctx = new (std::nothrow) care_context();
+    if (!ctx) return fail(CARE_ERR_DEVICE, "out of host memory");
+    try {
+        ctx->device = device;
+        ctx->opt = *opt;
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipDeviceGetAttribute(&ctx->numCUs, hipDeviceAttributeMultiprocessorCount, device));
+        ctx->cap = resultsPerMapThreshold(opt->coverage);
+        ctx->maxCand = opt->num_hash_maps * ctx->cap;
+        // quality weights: include/qualityscoreweights.hpp:15-30 (host powf)
+        float w[256];
+        for (int i = 0; i < 256; i++) {
+            float p = 1.0f;
+            if (i >= 33) p = std::pow(10.0f, float(-(i - 33)) / 10.0f);
+            w[i] = std::max(0.001f, 1.0f - p);
+        }
+        ctx->qweights.alloc(256);
+        HIP_TRY(hipMemcpy(ctx->qweights.p, w, sizeof(w), hipMemcpyHostToDevice));
+        ctx->counters.alloc(4);
+        ctx->rawHits.alloc(2);
+        ctx->hInts.alloc(8);
+        ctx->hU64.alloc(4);
+    } catch (const HipError& e) {
+        delete ctx;
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    } catch (const std::bad_alloc&) {
+        delete ctx;
+        return fail(CARE_ERR_DEVICE, "out of host memory");
+    }
+    *out = ctx;
+    return CARE_OK;
+}
+
+int care_context_destroy(care_context* ctx) {
+    if (!ctx) return CARE_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CARE_OK;
+}
+
+int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threads) {
+    if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
+    if (batch_anchors < 0 || (block_threads != 0 && block_threads != 64 && block_threads != 128 &&
+                              block_threads != 256 && block_threads != 512))
+        return fail(CARE_ERR_ARGUMENT, "invalid launch configuration");
+    if (batch_anchors > 0 && (batch_anchors & 1)) return fail(CARE_ERR_ARGUMENT, "batch_anchors must be even");
+    ctx->requestedBatch = batch_anchors;
+    ctx->blockThreads = block_threads ? block_threads : 256;
+    return CARE_OK;
+}
+
+int care_readstore_upload(care_context* ctx, uint64_t num_reads, int max_length, const uint32_t* seqs, int seq_pitch,
+                          const int32_t* lengths, const uint8_t* ambiguous, const uint32_t* quals, int qual_pitch,
+                          int qual_bits) {
+    if (!ctx || !seqs || !lengths) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (num_reads == 0 || num_reads >= 0xFFFFFFFFull) return fail(CARE_ERR_ARGUMENT, "num_reads out of range");
+    if (max_length < 1 || max_length > 1000) return fail(CARE_ERR_ARGUMENT, "max_length %d outside 1..1000", max_length);
+    if (seq_pitch < (max_length + 15) / 16) return fail(CARE_ERR_ARGUMENT, "seq_pitch too small");
+    if (ctx->opt.use_quality) {
+        if (!quals) return fail(CARE_ERR_ARGUMENT, "quality scores required with use_quality");
+        if (qual_bits != 1 && qual_bits != 2 && qual_bits != 8) return fail(CARE_ERR_ARGUMENT, "qual_bits must be 1, 2 or 8");
+        if (qual_pitch < (qual_bits * max_length + 31) / 32) return fail(CARE_ERR_ARGUMENT, "qual_pitch too small");
+    }
+    for (uint64_t i = 0; i < num_reads; i++)
+        if (lengths[i] < 1 || lengths[i] > max_length) return fail(CARE_ERR_ARGUMENT, "read %llu has length %d", (unsigned long long)i, lengths[i]);
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        ctx->seqs.alloc(num_reads * seq_pitch);
+        ctx->lens.alloc(num_reads);
+        ctx->ambig.alloc(num_reads);
+        HIP_TRY(hipMemcpy(ctx->seqs.p, seqs, sizeof(u32) * num_reads * seq_pitch, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ctx->lens.p, lengths, sizeof(int) * num_reads, hipMemcpyHostToDevice));
+        if (ambiguous) {
+            HIP_TRY(hipMemcpy(ctx->ambig.p, ambiguous, num_reads, hipMemcpyHostToDevice));
+        } else {
+            std::vector<u8> z(num_reads, 0);
+            HIP_TRY(hipMemcpy(ctx->ambig.p, z.data(), num_reads, hipMemcpyHostToDevice));
+        }
+        if (ctx->opt.use_quality) {
+            ctx->quals.alloc(num_reads * qual_pitch);
+            HIP_TRY(hipMemcpy(ctx->quals.p, quals, sizeof(u32) * num_reads * qual_pitch, hipMemcpyHostToDevice));
+        } else {
+            ctx->quals.alloc(1);
+            qual_pitch = 0;
+        }
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    ctx->rs.seqs = ctx->seqs.p;
+    ctx->rs.lens = ctx->lens.p;
+    ctx->rs.ambig = ctx->ambig.p;
+    ctx->rs.quals = ctx->quals.p;
+    ctx->rs.n = num_reads;
+    ctx->rs.pitch = seq_pitch;
+    ctx->rs.qpitch = qual_pitch;
+    ctx->rs.qbits = ctx->opt.use_quality ? qual_bits : 8;
+    ctx->rs.maxLen = max_length;
+    ctx->hasReads = true;
+    ctx->hasTables = false;
+    ctx->batchCap = 0;
+    return CARE_OK;
+}
+
+int care_readstore_gather(care_context* ctx, const uint32_t* ids, int n, uint32_t* out_seqs, int out_pitch,
+                          int32_t* out_lengths) {
+    if (!ctx || (n > 0 && !ids)) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");
+    if (out_seqs && out_pitch < ctx->rs.pitch) return fail(CARE_ERR_ARGUMENT, "out_pitch too small");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        for (int i = 0; i < n; i++) {
+            if (ids[i] >= ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "read id %u out of range", ids[i]);
+            if (out_seqs)
+                HIP_TRY(hipMemcpy(out_seqs + size_t(i) * out_pitch, ctx->seqs.p + size_t(ids[i]) * ctx->rs.pitch,
+                                  sizeof(u32) * ctx->rs.pitch, hipMemcpyDeviceToHost));
+            if (out_lengths)
+                HIP_TRY(hipMemcpy(out_lengths + i, ctx->lens.p + ids[i], sizeof(int), hipMemcpyDeviceToHost));
+        }
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_minhash_signatures(care_context* ctx, uint64_t first, int n, uint64_t* out) {
+    if (!ctx || !out) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");
+    if (n < 0 || first + uint64_t(n) > ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "range outside read store");
+    if (n == 0) return CARE_OK;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        DevBuf<u64> sig;
+        sig.alloc(size_t(n) * ctx->opt.num_hash_maps);
+        launch_signatures(ctx->rs, first, n, ctx->opt.kmer_length, ctx->opt.num_hash_maps, sig.p, ctx->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, sig.p, sizeof(u64) * size_t(n) * ctx->opt.num_hash_maps, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_minhash_build(care_context* ctx) {
+    if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
+    if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");
+    const int h = ctx->opt.num_hash_maps;
+    const u64 n = ctx->rs.n;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        DevBuf<u64> sigs;
+        sigs.alloc(n * h);
+        launch_signatures(ctx->rs, 0, int(n), ctx->opt.kmer_length, h, sigs.p, s);
+        HIP_TRY(hipGetLastError());
+        // scratch table at load <= 0.5
+        u64 tsize = 1;
+        while (tsize < 2 * n) tsize <<= 1;
+        DevBuf<u64> tkeys;
+        DevBuf<u32> tcount, tfill, toff;
+        tkeys.alloc(tsize);
+        tcount.alloc(tsize);
+        tfill.alloc(tsize);
+        toff.alloc(tsize);
+        DevBuf<unsigned long long> counters;
+        counters.alloc(2);
+        DevBuf<int> overflow;
+        overflow.alloc(1);
+        ctx->maps.clear();
+        std::vector<DevMap> hostMaps(h);
+        for (int m = 0; m < h; m++) {
+            launch_fill_u64(tkeys.p, tsize, kEmptyKey, s);
+            launch_fill_u32(tcount.p, tsize, 0, s);
+            launch_fill_u32(tfill.p, tsize, 0, s);
+            launch_fill_u64(reinterpret_cast<u64*>(counters.p), 2, 0, s);
+            launch_fill_u32(reinterpret_cast<u32*>(overflow.p), 1, 0, s);
+            launch_count_keys(sigs.p, n, h, m, tkeys.p, tcount.p, tsize - 1, overflow.p, s);
+            launch_assign_offsets(tkeys.p, tcount.p, tsize, ctx->cap, toff.p, counters.p, s);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(ctx->hU64.p, counters.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            const u64 numValues = ctx->hU64.p[0];
+            const u64 numKeys = ctx->hU64.p[1];
+            auto ms = std::make_unique<MapStore>();
+            u64 fsize = 1;
+            while (fsize < 2 * numKeys + 2) fsize <<= 1;
+            ms->keys.alloc(fsize);
+            ms->offcnt.alloc(fsize);
+            ms->values.alloc(numValues);
+            ms->mask = fsize - 1;
+            launch_fill_u64(ms->keys.p, fsize, kEmptyKey, s);
+            launch_insert_final(tkeys.p, tcount.p, toff.p, tsize, ms->keys.p, ms->offcnt.p, ms->mask, overflow.p, s);
+            launch_scatter_values(sigs.p, n, h, m, tkeys.p, toff.p, tsize - 1, tfill.p, ms->values.p, s);
+            launch_sort_buckets(toff.p, tcount.p, tsize, ms->values.p, s);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(ctx->hInts.p, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (ctx->hInts.p[0]) return fail(CARE_ERR_DEVICE, "hash table overflow while building map %d", m);
+            hostMaps[m] = DevMap{ms->keys.p, ms->offcnt.p, ms->values.p, ms->mask};
+            ctx->maps.push_back(std::move(ms));
+        }
+        ctx->devMaps.alloc(h);
+        HIP_TRY(hipMemcpy(ctx->devMaps.p, hostMaps.data(), sizeof(DevMap) * h, hipMemcpyHostToDevice));
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    ctx->tables.maps = ctx->devMaps.p;
+    ctx->tables.h = h;
+    ctx->tables.k = ctx->opt.kmer_length;
+    ctx->tables.kmask = ~u64(0) >> (2 * (32 - ctx->opt.kmer_length));
+    ctx->hasTables = true;
+    return CARE_OK;
+}
+
+int care_minhash_query(care_context* ctx, uint64_t first, int n, uint32_t* ids, int32_t* offsets,
+                       uint64_t ids_capacity) {
+    if (!ctx || !offsets || (ids_capacity > 0 && !ids)) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
+    if (n < 0 || first + uint64_t(n) > ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "range outside read store");
+    offsets[0] = 0;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int B = default_batch(ctx);
+        ensure_batch(ctx, B);
+        std::vector<int> counts;
+        std::vector<u32> buf;
+        uint64_t written = 0;
+        for (int done = 0; done < n; done += B) {
+            const int nb = std::min(B, n - done);
+            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->opt.exclude_ambiguous,
+                         ctx->candIds.p, ctx->candCount.p, nullptr, ctx->stream);
+            HIP_TRY(hipGetLastError());
+            counts.resize(nb);
+            buf.resize(size_t(nb) * ctx->maxCand);
+            HIP_TRY(hipMemcpyAsync(counts.data(), ctx->candCount.p, sizeof(int) * nb, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(buf.data(), ctx->candIds.p, sizeof(u32) * size_t(nb) * ctx->maxCand,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            for (int i = 0; i < nb; i++) {
+                const int c = counts[i];
+                if (written + c > ids_capacity) return fail(CARE_ERR_CAPACITY, "ids_capacity exceeded");
+                std::memcpy(ids + written, buf.data() + size_t(i) * ctx->maxCand, sizeof(u32) * c);
+                written += c;
+                offsets[done + i + 1] = int32_t(written);
+            }
+        }
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
+    if (!ctx || !r || !r->anchor_status || !r->anchor_sequences) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
+    if (n < 0 || first + uint64_t(n) > ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "range outside read store");
+    if (r->sequence_pitch < ctx->rs.maxLen) return fail(CARE_ERR_ARGUMENT, "sequence_pitch < max read length");
+    if (ctx->opt.paired_end && ((first & 1) || (n & 1)))
+        return fail(CARE_ERR_ARGUMENT, "paired-end batches must start at an even id and hold whole pairs");
+    if (ctx->opt.correct_candidates && (!r->candidate_read_ids || !r->candidate_sequences))
+        return fail(CARE_ERR_ARGUMENT, "candidate output buffers required with correct_candidates");
+    r->num_candidates = 0;
+    ctx->stats = care_batch_stats{};
+    const auto& o = ctx->opt;
+    CorrectParams P{};
+    P.rs = ctx->rs;
+    P.maxCand = ctx->maxCand;
+    P.qweights = ctx->qweights.p;
+    P.pairedEnd = o.paired_end;
+    P.useQuality = o.use_quality;
+    P.correctCandidates = o.correct_candidates;
+    P.newColumns = o.new_columns_to_correct;
+    P.minOverlap = o.min_overlap;
+    P.maxErrorRate = o.max_mismatch_ratio;
+    P.minOverlapRatio = o.min_overlap_ratio;
+    P.maxErr4 = o.error_factor * 4.0f;
+    const float base = o.error_factor * 1.0f;
+    P.mmBase2 = 2 * base;
+    P.mmBase3 = 3 * base;
+    P.mmBase4 = 4 * base;
+    P.goodAlignmentsThreshold = o.coverage * o.coverage_factor;
+    P.pairedThreshold = o.paired_filter_threshold;
+    P.sigCountThreshold = int(float(int(o.coverage)) * 0.3f);  // significance count, src/msa.cpp:537-541
+    P.avgSupportThreshold = 1.0f - o.error_factor;
+    P.minSupportThreshold = std::fma(-3.0f, o.error_factor, 1.0f);
+    P.minCoverageThreshold = o.coverage_factor / 6.0f * o.coverage;
+    P.errPercent = int(std::ceil(o.error_factor * 100.0f));
+    P.errPercentHalf = float(P.errPercent) * 0.5f;
+    P.correctionType = o.correction_type;
+    P.correctionTypeCands = o.correction_type_cands;
+    P.thresholdAnchor = o.threshold_anchor;
+    P.thresholdCands = o.threshold_cands;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int B = default_batch(ctx);
+        ensure_batch(ctx, B);
+        const int T = ctx->blockThreads;
+        int blocksPerCU = 0;
+        const size_t lds = correct_lds_bytes(ctx->rs.maxLen, T);
+        blocksPerCU = int(std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds + 1024))));
+        hipStream_t s = ctx->stream;
+        P.candIds = ctx->candIds.p;
+        P.candCount = ctx->candCount.p;
+        P.metaA = ctx->metaA.p;
+        P.metaB = ctx->metaB.p;
+        P.outStatus = ctx->outStatus.p;
+        P.outSeq = ctx->outSeq.p;
+        P.outPitch = ctx->outPitch;
+        P.candOutIds = ctx->candOutIds.p;
+        P.candOutSeq = ctx->candOutSeq.p;
+        P.candOutCount = ctx->counters.p + 1;
+        P.candOutCap = ctx->candOutCap;
+        P.queueHead = ctx->counters.p;
+        float qms = 0, cms = 0;
+        HIP_TRY(hipEventRecord(ctx->ev[2], s));
+        for (int done = 0; done < n; done += B) {
+            const int nb = std::min(B, n - done);
+            launch_fill_u64(reinterpret_cast<u64*>(ctx->rawHits.p), 2, 0, s);
+            HIP_TRY(hipEventRecord(ctx->ev[0], s));
+            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, o.exclude_ambiguous, ctx->candIds.p,
+                         ctx->candCount.p, ctx->rawHits.p, s);
+            HIP_TRY(hipEventRecord(ctx->ev[1], s));
+            launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);
+            P.firstAnchor = first + done;
+            P.numAnchors = nb;
+            const int blocks = std::max(1, std::min(nb, ctx->numCUs * blocksPerCU));
+            launch_correct(P, blocks, T, s);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(ctx->ev[3], s));
+            HIP_TRY(hipMemcpyAsync(ctx->hStatus.p, ctx->outStatus.p, nb, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(ctx->hSeq.p, ctx->outSeq.p, size_t(nb) * ctx->outPitch, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(ctx->hInts.p, ctx->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(ctx->hU64.p, ctx->rawHits.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            float t1 = 0, t2 = 0;
+            HIP_TRY(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
+            HIP_TRY(hipEventElapsedTime(&t2, ctx->ev[1], ctx->ev[3]));
+            qms += t1;
+            cms += t2;
+            ctx->stats.raw_hits += ctx->hU64.p[0];
+            ctx->stats.candidates += ctx->hU64.p[1];
+            for (int i = 0; i < nb; i++) {
+                r->anchor_status[done + i] = ctx->hStatus.p[i];
+                if (ctx->hStatus.p[i] & 1)
+                    std::memcpy(r->anchor_sequences + size_t(done + i) * r->sequence_pitch, ctx->hSeq.p + size_t(i) * ctx->outPitch,
+                                ctx->rs.maxLen);
+            }
+            if (o.correct_candidates) {
+                const int nc = ctx->hInts.p[1];
+                if (nc > ctx->candOutCap) return fail(CARE_ERR_CAPACITY, "candidate correction buffer overflow (%d)", nc);
+                if (r->num_candidates + nc > r->candidate_capacity)
+                    return fail(CARE_ERR_CAPACITY, "candidate_capacity exceeded");
+                HIP_TRY(hipMemcpyAsync(ctx->hCandIds.p, ctx->candOutIds.p, sizeof(u32) * nc, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipMemcpyAsync(ctx->hCandSeq.p, ctx->candOutSeq.p, size_t(nc) * ctx->outPitch, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+                for (int i = 0; i < nc; i++) {
+                    r->candidate_read_ids[r->num_candidates + i] = ctx->hCandIds.p[i];
+                    std::memcpy(r->candidate_sequences + size_t(r->num_candidates + i) * r->sequence_pitch,
+                                ctx->hCandSeq.p + size_t(i) * ctx->outPitch, ctx->rs.maxLen);
+                }
+                r->num_candidates += nc;
+            }
+        }
+        HIP_TRY(hipEventRecord(ctx->ev[3], s));
+        HIP_TRY(hipEventSynchronize(ctx->ev[3]));
+        float tot = 0;
+        HIP_TRY(hipEventElapsedTime(&tot, ctx->ev[2], ctx->ev[3]));
+        ctx->stats.query_ms = qms;
+        ctx->stats.correct_ms = cms;
+        ctx->stats.total_ms = tot;
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_last_batch_stats(care_context* ctx, care_batch_stats* stats) {
+    if (!ctx || !stats) return fail(CARE_ERR_ARGUMENT, "null argument");
+    *stats = ctx->stats;
+    return CARE_OK;
+}
+
+int care_forest_load(care_context* ctx, int which, const char* path, uint32_t max_trees) {
+    (void)which;
+    (void)path;
+    (void)max_trees;
+    if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
+    return fail(CARE_ERR_STATE, "forest classifier not available in this build");
+}
+
+}  // extern "C"

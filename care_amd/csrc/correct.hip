@@ -1,0 +1,664 @@
+// Fused per-anchor correction kernel: shifted-Hamming alignment, orientation
+// choice, mismatch-ratio / pair filters, MSA build, different-region
+// refinement, MSA properties, anchor correction and candidate correction.
+//
+// One workgroup corrects one anchor at a time and pulls anchors from a
+// device-side work queue (persistent kernel), so anchors with very different
+// candidate counts balance across the chip.  Semantics follow the reference
+// CPU corrector exactly (include/corrector.hpp:145-536, src/msa.cpp,
+// src/cpu_alignment.cpp, include/alignmentorientation.hpp), which the GPU
+// reference does not (it subtracts removed candidates during refinement,
+// include/gpu/gpumsa.cuh:743-812).  Floating-point sums run in the CPU's
+// order: per column, anchor first, then candidates in ascending index.
+
+#include "device_common.h"
+
+namespace care_engine {
+
+namespace {
+
+__device__ __forceinline__ bool feq(float l, float r) {
+    const float d = l - r < 0 ? -(l - r) : l - r;
+    return d < 1e-5f;
+}
+__device__ __forceinline__ bool fleq(float l, float r) { return (l < r) || feq(l, r); }
+__device__ __forceinline__ bool fgeq(float l, float r) { return (l > r) || feq(l, r); }
+
+// 32 bits of a MSB-first bit plane starting at bit `pos` (plane padded with
+// at least one zero word past its end)
+__device__ __forceinline__ u32 fetch_bits(const u32* p, int pos) {
+    const int w = pos >> 5, s = pos & 31;
+    const u32 x = p[w];
+    return s == 0 ? x : (x << s) | (p[w + 1] >> (32 - s));
+}
+
+// pack the even bits (0,2,..,30) of x into the low 16 bits
+__device__ __forceinline__ u32 even_bits(u32 x) {
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    x = (x | (x >> 8)) & 0x0000FFFFu;
+    return x;
+}
+
+struct Lds {
+    u32* aw;        // anchor 2-bit words [pitch]
+    u32* ahi;       // anchor planes [hw + 2]
+    u32* alo;
+    char* anchorQ;  // [maxLen]
+    u32* planes;    // per wave: 4 planes of [hw + 2]
+    u8* cons;       // [cols]
+    float* support;
+    int* coverage;
+    int* origCov;
+    u16* cnt;       // [4][cols]
+    float* wgt;     // [4][cols]
+    int* scratch;   // [32]
+};
+
+struct LdsDims {
+    int pitch, hw, maxLen, cols, waves;
+    __host__ __device__ size_t bytes() const {
+        size_t b = 0;
+        b += sizeof(u32) * pitch;
+        b += 2 * sizeof(u32) * (hw + 2);
+        b += (size_t(maxLen) + 3) / 4 * 4;
+        b += size_t(waves) * 4 * sizeof(u32) * (hw + 2);
+        b += (size_t(cols) + 3) / 4 * 4;
+        b += sizeof(float) * cols + 2 * sizeof(int) * cols;
+        b += 4 * sizeof(u16) * cols + 4 * sizeof(float) * cols;
+        b += 32 * sizeof(int);
+        return b;
+    }
+};
+
+__device__ Lds carve(char* base, const LdsDims& d) {
+    Lds l;
+    char* p = base;
+    l.aw = reinterpret_cast<u32*>(p); p += sizeof(u32) * d.pitch;
+    l.ahi = reinterpret_cast<u32*>(p); p += sizeof(u32) * (d.hw + 2);
+    l.alo = reinterpret_cast<u32*>(p); p += sizeof(u32) * (d.hw + 2);
+    l.planes = reinterpret_cast<u32*>(p); p += size_t(d.waves) * 4 * sizeof(u32) * (d.hw + 2);
+    l.support = reinterpret_cast<float*>(p); p += sizeof(float) * d.cols;
+    l.coverage = reinterpret_cast<int*>(p); p += sizeof(int) * d.cols;
+    l.origCov = reinterpret_cast<int*>(p); p += sizeof(int) * d.cols;
+    l.wgt = reinterpret_cast<float*>(p); p += 4 * sizeof(float) * d.cols;
+    l.scratch = reinterpret_cast<int*>(p); p += 32 * sizeof(int);
+    l.cnt = reinterpret_cast<u16*>(p); p += 4 * sizeof(u16) * d.cols;
+    l.cons = reinterpret_cast<u8*>(p); p += (size_t(d.cols) + 3) / 4 * 4;
+    l.anchorQ = reinterpret_cast<char*>(p);
+    return l;
+}
+
+struct Msa {
+    int cols;
+    int ab;  // first anchor column
+};
+
+// Builds the MSA of the anchor and candidates meta[0..m) into LDS
+// (src/msa.cpp:9-173).  All threads must call.
+__device__ Msa build_msa(const CorrectParams& P, const Lds& L, const CandMeta* meta, int m, int la, int* sMin,
+                         int* sMax) {
+    if (threadIdx.x == 0) {
+        *sMin = 0;
+        *sMax = la;
+    }
+    __syncthreads();
+    int lmin = 0, lmax = la;
+    for (int c = threadIdx.x; c < m; c += blockDim.x) {
+        const int s = meta[c].shift;
+        lmin = min(lmin, s);
+        lmax = max(lmax, s + int(meta[c].len));
+    }
+    atomicMin(sMin, lmin);
+    atomicMax(sMax, lmax);
+    __syncthreads();
+    Msa msa;
+    msa.cols = *sMax - *sMin;
+    msa.ab = -*sMin;
+    const int ae = msa.ab + la;
+    const DevReadStore& rs = P.rs;
+    const bool useQ = P.useQuality != 0;
+    const int cols = msa.cols;
+    for (int col = threadIdx.x; col < cols; col += blockDim.x) {
+        int cA = 0, cC = 0, cG = 0, cT = 0;
+        float wA = 0.f, wC = 0.f, wG = 0.f, wT = 0.f;
+        auto add = [&](u32 b, float w) {
+            switch (b) {
+                case 0: cA++; wA += w; break;
+                case 1: cC++; wC += w; break;
+                case 2: cG++; wG += w; break;
+                default: cT++; wT += w; break;
+            }
+        };
+        u32 anchorBase = 0;
+        if (col >= msa.ab && col < ae) {
+            const int i = col - msa.ab;
+            anchorBase = base_at(L.aw, i);
+            const float w = useQ ? P.qweights[(unsigned char)L.anchorQ[i]] : 1.0f;
+            add(anchorBase, 1.0f * w);
+        }
+        for (int c = 0; c < m; c++) {
+            const CandMeta cm = meta[c];
+            const int pos = col - msa.ab - int(cm.shift);
+            if (pos < 0 || pos >= int(cm.len)) continue;
+            const bool rc = cm.flag == 2;
+            const u32* enc = rs.seqs + u64(cm.id) * rs.pitch;
+            const u32 b = oriented_base(enc, cm.len, pos, rc);
+            float w = cm.u.weight;
+            if (useQ) {
+                const int qi = rc ? rs.maxLen - 1 - pos : pos;
+                const char q = quality_char(rs.quals + u64(cm.id) * rs.qpitch, qi, rs.qbits);
+                w = cm.u.weight * P.qweights[(unsigned char)q];
+            }
+            add(b, w);
+        }
+        u8 cons = 0;
+        float cw = wA;
+        if (wC > cw) { cons = 1; cw = wC; }
+        if (wG > cw) { cons = 2; cw = wG; }
+        if (wT > cw) { cons = 3; cw = wT; }
+        const float colw = wA + wC + wG + wT;
+        L.cons[col] = cons;
+        L.support[col] = cw / colw;
+        L.coverage[col] = cA + cC + cG + cT;
+        L.cnt[0 * cols + col] = u16(cA);
+        L.cnt[1 * cols + col] = u16(cC);
+        L.cnt[2 * cols + col] = u16(cG);
+        L.cnt[3 * cols + col] = u16(cT);
+        L.wgt[0 * cols + col] = wA;
+        L.wgt[1 * cols + col] = wC;
+        L.wgt[2 * cols + col] = wG;
+        L.wgt[3 * cols + col] = wT;
+        if (col >= msa.ab && col < ae) {
+            const int oc[4] = {cA, cC, cG, cT};
+            L.origCov[col] = oc[anchorBase];
+        }
+    }
+    __syncthreads();
+    return msa;
+}
+
+// Alignment of the anchor against the candidate planes of one wave, all
+// shifts in parallel; result key = score << 11 | order index, or ~0u.
+__device__ __forceinline__ void align_wave(const u32* ahi, const u32* alo, int la, const u32* fhi, const u32* flo,
+                                           const u32* rhi, const u32* rlo, int lc, int minov, float maxErrorRate,
+                                           u32* keyF, u32* keyR) {
+    const int lane = lane_id();
+    const int nPos = max(0, la - minov + 1);
+    const int nNeg = max(0, lc - minov);
+    const int tot = nPos + nNeg;
+    const int total = la + lc;
+    u32 kf = ~0u, kr = ~0u;
+    for (int t = lane; t < 2 * tot; t += kWave) {
+        const bool isRc = t >= tot;
+        const int u = isRc ? t - tot : t;
+        const int s = u < nPos ? u : -(u - nPos + 1);
+        const int ov = s >= 0 ? min(la - s, lc) : min(la, lc + s);
+        const int lim = int(float(ov) * maxErrorRate);
+        if (lim <= 0) continue;
+        const int sa = s > 0 ? s : 0;
+        const int sc = s < 0 ? -s : 0;
+        const u32* chi = isRc ? rhi : fhi;
+        const u32* clo = isRc ? rlo : flo;
+        int mm = 0;
+        for (int off = 0; off < ov && mm < lim; off += 32) {
+            const int nb = ov - off < 32 ? ov - off : 32;
+            u32 x = (fetch_bits(ahi, sa + off) ^ fetch_bits(chi, sc + off)) |
+                    (fetch_bits(alo, sa + off) ^ fetch_bits(clo, sc + off));
+            if (nb < 32) x &= ~0u << (32 - nb);
+            mm += __popc(x);
+        }
+        if (mm < lim) {
+            const u32 key = (u32(mm + total - 2 * ov) << 11) | u32(u);
+            if (isRc) kr = key < kr ? key : kr;
+            else kf = key < kf ? key : kf;
+        }
+    }
+    *keyF = wave_min_u32(kf);
+    *keyR = wave_min_u32(kr);
+}
+
+struct Aln {
+    int overlap, shift, nOps;
+    bool valid;
+};
+
+__device__ __forceinline__ Aln decode_key(u32 key, int la, int lc, int minov) {
+    Aln a;
+    const int total = la + lc;
+    if (key == ~0u) {
+        a.valid = false;
+        a.shift = -lc;
+        a.overlap = 0;
+        a.nOps = 0;
+        return a;
+    }
+    const int nPos = max(0, la - minov + 1);
+    const int u = int(key & 2047u);
+    const int score = int(key >> 11);
+    a.shift = u < nPos ? u : -(u - nPos + 1);
+    const int cb = max(-a.shift, 0);
+    const int ce = min(lc, la - a.shift);
+    a.overlap = ce - cb;
+    a.nOps = score - total + 2 * a.overlap;
+    a.valid = true;
+    return a;
+}
+
+// include/alignmentorientation.hpp:14-70; maxErr = 4 * estimatedErrorrate
+__device__ __forceinline__ int choose_orientation(const Aln& f, const Aln& r, int la, float minOverlapRatio,
+                                                  int minOverlap, float maxErr) {
+    const int prod = int(float(la) * minOverlapRatio);
+    const int minov = prod > minOverlap ? prod : minOverlap;
+    if (f.valid && f.overlap >= minov) {
+        if (r.valid && r.overlap >= minov) {
+            const float ratio = float(f.nOps) / float(f.overlap);
+            const float rratio = float(r.nOps) / float(r.overlap);
+            if (ratio < rratio) return ratio < maxErr ? 1 : 3;
+            if (rratio < ratio) return rratio < maxErr ? 2 : 3;
+            if (ratio < maxErr) return f.overlap > r.overlap ? 1 : 2;
+            return 3;
+        }
+        return float(f.nOps) / float(f.overlap) < maxErr ? 1 : 3;
+    }
+    if (r.valid && r.overlap >= minov && float(r.nOps) / float(r.overlap) < maxErr) return 2;
+    return 3;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) {
+    HIP_DYNAMIC_SHARED(char, ldsRaw)
+    const Lds L = carve(ldsRaw, dims);
+    __shared__ int sAnchor, sMin, sMax, sCount[3], sFlag, sCol, sTotal;
+    __shared__ float sThr;
+    const DevReadStore& rs = P.rs;
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid / kWave;
+    const int nwaves = blockDim.x / kWave;
+    const int hw2 = dims.hw + 2;
+    u32* myPlanes = L.planes + size_t(wave) * 4 * hw2;
+    u32* fhi = myPlanes;
+    u32* flo = myPlanes + hw2;
+    u32* rhi = myPlanes + 2 * hw2;
+    u32* rlo = myPlanes + 3 * hw2;
+
+    for (;;) {
+        if (tid == 0) sAnchor = atomicAdd(P.queueHead, 1);
+        __syncthreads();
+        const int a = sAnchor;
+        __syncthreads();
+        if (a >= P.numAnchors) break;
+
+        const u64 id = P.firstAnchor + u64(a);
+        const int la = rs.lens[id];
+        const int n = P.candCount[a];
+        const u32* ids = P.candIds + u64(a) * P.maxCand;
+        u8 status = 0;
+        char* outSeq = P.outSeq + u64(a) * P.outPitch;
+
+        if (n == 0) {
+            if (tid == 0) P.outStatus[a] = 0;
+            continue;
+        }
+
+        // anchor words, planes and qualities
+        const u32* aenc = rs.seqs + id * rs.pitch;
+        for (int i = tid; i < rs.pitch; i += blockDim.x) L.aw[i] = aenc[i];
+        if (P.useQuality) {
+            const u32* q = rs.quals + id * rs.qpitch;
+            for (int i = tid; i < la; i += blockDim.x) L.anchorQ[i] = quality_char(q, i, rs.qbits);
+        }
+        __syncthreads();
+        for (int w = tid; w < hw2; w += blockDim.x) {
+            u32 hi = 0, lo = 0;
+            if (2 * w < rs.pitch) {
+                const u32 x0 = L.aw[2 * w];
+                const u32 x1 = 2 * w + 1 < rs.pitch ? L.aw[2 * w + 1] : 0u;
+                hi = (even_bits(x0 >> 1) << 16) | even_bits(x1 >> 1);
+                lo = (even_bits(x0) << 16) | even_bits(x1);
+                // clear bits beyond the read
+                const int valid = la - 32 * w;
+                if (valid <= 0) { hi = 0; lo = 0; }
+                else if (valid < 32) {
+                    const u32 mask = ~0u << (32 - valid);
+                    hi &= mask;
+                    lo &= mask;
+                }
+            }
+            L.ahi[w] = hi;
+            L.alo[w] = lo;
+        }
+        __syncthreads();
+
+        // ---- phase A: alignments, one wave per candidate ----
+        const int minovA = max(P.minOverlap, int(float(la) * P.minOverlapRatio));
+        const u32* mateIds = nullptr;
+        int mateN = 0;
+        if (P.pairedEnd) {
+            const int mate = a ^ 1;
+            if (mate < P.numAnchors) {
+                mateIds = P.candIds + u64(mate) * P.maxCand;
+                mateN = P.candCount[mate];
+            }
+        }
+        CandMeta* metaA = P.metaA + u64(a) * P.maxCand;
+        CandMeta* metaB = P.metaB + u64(a) * P.maxCand;
+        for (int c = wave; c < n; c += nwaves) {
+            const u32 cid = ids[c];
+            const int lc = rs.lens[cid];
+            const u32* cenc = rs.seqs + u64(cid) * rs.pitch;
+            const int cw = (lc + 31) / 32;
+            for (int w = lane; w < hw2; w += kWave) {
+                u32 hi = 0, lo = 0;
+                if (w < cw) {
+                    const u32 x0 = cenc[2 * w];
+                    const u32 x1 = 2 * w + 1 < rs.pitch ? cenc[2 * w + 1] : 0u;
+                    hi = (even_bits(x0 >> 1) << 16) | even_bits(x1 >> 1);
+                    lo = (even_bits(x0) << 16) | even_bits(x1);
+                    const int valid = lc - 32 * w;
+                    if (valid < 32) {
+                        const u32 mask = ~0u << (32 - valid);
+                        hi &= mask;
+                        lo &= mask;
+                    }
+                }
+                fhi[w] = hi;
+                flo[w] = lo;
+            }
+            wave_sync();
+            // reverse complement planes: rc[q] = 1 - f[lc-1-q]
+            for (int w = lane; w < hw2; w += kWave) {
+                u32 hi = 0, lo = 0;
+                if (w < cw) {
+                    const int e = lc - 32 * (w + 1);  // first fwd bit of the window
+                    u32 xh, xl;
+                    if (e >= 0) {
+                        xh = fetch_bits(fhi, e);
+                        xl = fetch_bits(flo, e);
+                    } else {
+                        xh = fhi[0] >> (-e);
+                        xl = flo[0] >> (-e);
+                    }
+                    hi = ~__brev(xh);
+                    lo = ~__brev(xl);
+                    const int valid = lc - 32 * w;
+                    if (valid < 32) {
+                        const u32 mask = ~0u << (32 - valid);
+                        hi &= mask;
+                        lo &= mask;
+                    }
+                }
+                rhi[w] = hi;
+                rlo[w] = lo;
+            }
+            wave_sync();
+            u32 kf, kr;
+            align_wave(L.ahi, L.alo, la, fhi, flo, rhi, rlo, lc, minovA, P.maxErrorRate, &kf, &kr);
+            if (lane == 0) {
+                const Aln f = decode_key(kf, la, lc, minovA);
+                const Aln r = decode_key(kr, la, lc, minovA);
+                const int flag = choose_orientation(f, r, la, P.minOverlapRatio, P.minOverlap, P.maxErr4);
+                const Aln& ch = flag == 2 ? r : f;
+                CandMeta m;
+                m.id = cid;
+                m.shift = short(ch.shift);
+                m.len = u16(lc);
+                m.u.al.nOps = u16(ch.nOps);
+                m.u.al.overlap = u16(ch.overlap);
+                m.flag = u8(flag);
+                u8 paired = 0;
+                if (mateIds) {
+                    const u32 want = cid ^ 1u;
+                    int lo2 = 0, hi2 = mateN;
+                    while (lo2 < hi2) {
+                        const int mid = (lo2 + hi2) >> 1;
+                        if (mateIds[mid] < want) lo2 = mid + 1;
+                        else hi2 = mid;
+                    }
+                    paired = (lo2 < mateN && mateIds[lo2] == want) ? 1 : 0;
+                }
+                m.paired = paired;
+                m.pad = 0;
+                metaA[c] = m;
+            }
+            wave_sync();
+        }
+        __syncthreads();
+
+        // ---- phase B: orientation + mismatch-ratio filters ----
+        if (!P.pairedEnd) {
+            if (tid < 3) sCount[tid] = 0;
+            __syncthreads();
+            int c0 = 0, c1 = 0, c2 = 0;
+            for (int c = tid; c < n; c += blockDim.x) {
+                const CandMeta m = metaA[c];
+                if (m.flag == 3) continue;
+                const float ratio = float(m.u.al.nOps) / float(m.u.al.overlap);
+                c0 += ratio < P.mmBase2;
+                c1 += ratio < P.mmBase3;
+                c2 += ratio < P.mmBase4;
+            }
+            c0 = wave_sum_int(c0);
+            c1 = wave_sum_int(c1);
+            c2 = wave_sum_int(c2);
+            if (lane == 0) {
+                atomicAdd(&sCount[0], c0);
+                atomicAdd(&sCount[1], c1);
+                atomicAdd(&sCount[2], c2);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                float thr = -3.40282347e38f;  // numeric_limits<float>::lowest()
+                if (sCount[0] > 0 || sCount[1] > 0 || sCount[2] > 0) {
+                    if (float(sCount[0]) >= P.goodAlignmentsThreshold) thr = P.mmBase2;
+                    else if (float(sCount[1]) >= P.goodAlignmentsThreshold) thr = P.mmBase3;
+                    else if (float(sCount[2]) >= P.goodAlignmentsThreshold) thr = P.mmBase4;
+                }
+                sThr = thr;
+            }
+            __syncthreads();
+        }
+        int m = 0;
+        for (int base = 0; base < n; base += blockDim.x) {
+            const int c = base + tid;
+            bool keep = false;
+            CandMeta cm{};
+            if (c < n) {
+                cm = metaA[c];
+                if (cm.flag != 3) {
+                    const float ratio = float(cm.u.al.nOps) / float(cm.u.al.overlap);
+                    keep = P.pairedEnd ? (cm.paired || ratio < P.pairedThreshold) : (ratio < sThr);
+                }
+            }
+            int cnt;
+            const int pos = block_exclusive_count(keep, L.scratch, &cnt);
+            if (keep) {
+                const float w = 1.0f - sqrtf(float(cm.u.al.nOps) / (float(cm.u.al.overlap) * P.maxErrorRate));
+                cm.u.weight = w;
+                metaB[m + pos] = cm;
+            }
+            m += cnt;
+        }
+        __syncthreads();
+        if (m == 0) {
+            if (tid == 0) P.outStatus[a] = 0;
+            continue;
+        }
+
+        // ---- phase C: MSA + refinement (include/corrector.hpp:1307-1421) ----
+        CandMeta* cur = metaB;
+        CandMeta* other = metaA;
+        Msa msa = build_msa(P, L, cur, m, la, &sMin, &sMax);
+        for (int iter = 0; iter < 5 && m > 0; iter++) {
+            // src/msa.cpp:532-734
+            if (tid == 0) {
+                sFlag = 0;
+                sCol = 0x7FFFFFFF;
+            }
+            __syncthreads();
+            for (int i = tid; i < la; i += blockDim.x)
+                if (L.cons[msa.ab + i] != u8(base_at(L.aw, i))) sFlag = 1;
+            __syncthreads();
+            if (!sFlag) break;
+            for (int i = tid; i < la; i += blockDim.x) {
+                const int col = msa.ab + i;
+                const int ci = L.cons[col];
+                bool found = false;
+                for (int b = 0; b < 4; b++)
+                    if (b != ci && P.sigCountThreshold <= int(L.cnt[b * msa.cols + col])) found = true;
+                if (found) atomicMin(&sCol, col);
+            }
+            __syncthreads();
+            const int col = sCol;
+            if (col == 0x7FFFFFFF) break;
+            const int ci = L.cons[col];
+            int foundIdx = -1;
+            for (int b = 0; b < 4; b++)
+                if (b != ci && P.sigCountThreshold <= int(L.cnt[b * msa.cols + col])) foundIdx = b;
+            const u32 origBase = base_at(L.aw, col - msa.ab);
+            const bool keepMatching = origBase == u32(foundIdx);
+            if (tid == 0) sFlag = 0;  // very good alignment among removed
+            __syncthreads();
+            int removed = 0;
+            for (int c = tid; c < m; c += blockDim.x) {
+                const CandMeta cm = cur[c];
+                const int rb = msa.ab + cm.shift;
+                const int re = rb + cm.len;
+                bool rem = false;
+                if (col >= rb && col < re) {
+                    const u32 b = oriented_base(rs.seqs + u64(cm.id) * rs.pitch, cm.len, col - rb, cm.flag == 2);
+                    rem = keepMatching ? (b != u32(foundIdx)) : (b == u32(foundIdx));
+                }
+                if (rem && cm.u.weight >= 0.90f) sFlag = 1;
+                removed += rem;
+            }
+            removed = wave_sum_int(removed);
+            if (lane == 0 && removed) atomicAdd(&sFlag, 2);
+            __syncthreads();
+            if ((sFlag & 1) || sFlag == 0) break;  // nothing removed: MSA unchanged
+            int m2 = 0;
+            for (int base = 0; base < m; base += blockDim.x) {
+                const int c = base + tid;
+                bool keep = false;
+                CandMeta cm{};
+                if (c < m) {
+                    cm = cur[c];
+                    const int rb = msa.ab + cm.shift;
+                    const int re = rb + cm.len;
+                    keep = true;
+                    if (col >= rb && col < re) {
+                        const u32 b = oriented_base(rs.seqs + u64(cm.id) * rs.pitch, cm.len, col - rb, cm.flag == 2);
+                        keep = !(keepMatching ? (b != u32(foundIdx)) : (b == u32(foundIdx)));
+                    }
+                }
+                int cnt;
+                const int pos = block_exclusive_count(keep, L.scratch, &cnt);
+                if (keep) other[m2 + pos] = cm;
+                m2 += cnt;
+            }
+            __syncthreads();
+            CandMeta* t = cur;
+            cur = other;
+            other = t;
+            m = m2;
+            msa = build_msa(P, L, cur, m, la, &sMin, &sMax);
+        }
+
+        // ---- phase D: anchor correction (src/msa.cpp:304-453) ----
+        if (m > 0) {
+            __shared__ float sMinSupport, sAvgSupport;
+            __shared__ int sMinCov, sHQ, sByCons;
+            if (tid == 0) {
+                float mn = L.support[msa.ab];
+                float sum = 0.0f;
+                int mc = L.coverage[msa.ab];
+                for (int i = 0; i < la; i++) {
+                    const float s = L.support[msa.ab + i];
+                    mn = s < mn ? s : mn;
+                    sum += s;
+                    const int cv = L.coverage[msa.ab + i];
+                    mc = cv < mc ? cv : mc;
+                }
+                const float avg = sum / float(la);
+                const bool allGood = fgeq(avg, P.avgSupportThreshold) && fgeq(mn, P.minSupportThreshold) &&
+                                     fgeq(float(mc), P.minCoverageThreshold);
+                int hq = 0;
+                if (allGood) {
+                    int smallest = 100;
+                    for (int percent = P.errPercent; percent >= 0; percent--) {
+                        const float factor = float(percent) / 100.0f;
+                        if (fgeq(avg, 1.0f - factor) && fgeq(mn, fmaf(-3.0f, factor, 1.0f))) smallest = percent;
+                    }
+                    hq = fgeq(float(mc), P.minCoverageThreshold) && fleq(float(smallest), P.errPercentHalf);
+                }
+                sMinSupport = mn;
+                sAvgSupport = avg;
+                sMinCov = mc;
+                sHQ = hq;
+                sByCons = allGood;
+            }
+            __syncthreads();
+            const bool byCons = sByCons != 0;
+            for (int i = tid; i < la; i += blockDim.x) {
+                const int col = msa.ab + i;
+                u32 b;
+                if (byCons) b = L.cons[col];
+                else b = (L.support[col] > 0.90f && L.origCov[col] <= 2) ? u32(L.cons[col]) : base_at(L.aw, i);
+                outSeq[i] = "ACGT"[b];
+            }
+            status = u8(1 | (sHQ ? 2 : 0));
+
+            // ---- phase E: candidate correction (src/msa.cpp:457-522) ----
+            if (sHQ && P.correctCandidates) {
+                const int ae = msa.ab + la;
+                for (int c = wave; c < m; c += nwaves) {
+                    const CandMeta cm = cur[c];
+                    const int qb = msa.ab + cm.shift;
+                    const int qe = qb + cm.len;
+                    if (!(msa.ab - P.newColumns <= qb && qb <= msa.ab + P.newColumns && qe <= ae + P.newColumns)) continue;
+                    int slot = 0;
+                    if (lane == 0) slot = atomicAdd(P.candOutCount, 1);
+                    slot = __shfl(slot, 0, kWave);
+                    if (slot >= P.candOutCap) continue;
+                    if (lane == 0) P.candOutIds[slot] = cm.id;
+                    char* dst = P.candOutSeq + u64(slot) * P.outPitch;
+                    const bool rc = cm.flag == 2;
+                    for (int i = lane; i < cm.len; i += kWave) {
+                        // forward strand of the corrected candidate
+                        const u32 b = rc ? 3u - L.cons[qb + cm.len - 1 - i] : L.cons[qb + i];
+                        dst[i] = "ACGT"[b];
+                    }
+                }
+            }
+        }
+        if (tid == 0) P.outStatus[a] = status;
+        __syncthreads();
+    }
+}
+
+size_t correct_lds_bytes(int maxLen, int threads) {
+    LdsDims d;
+    d.pitch = (maxLen + 15) / 16;
+    d.hw = (maxLen + 31) / 32;
+    d.maxLen = maxLen;
+    d.cols = 3 * maxLen + 2;
+    d.waves = threads / kWave;
+    return d.bytes();
+}
+
+void launch_correct(const CorrectParams& p, int blocks, int threads, hipStream_t stream) {
+    LdsDims d;
+    d.pitch = p.rs.pitch;
+    d.hw = (p.rs.maxLen + 31) / 32;
+    d.maxLen = p.rs.maxLen;
+    d.cols = 3 * p.rs.maxLen + 2;
+    d.waves = threads / kWave;
+    const size_t lds = d.bytes();
+    hipLaunchKernelGGL(k_correct, dim3(blocks), dim3(threads), lds, stream, p, d);
+}
+
+}  // namespace care_engine

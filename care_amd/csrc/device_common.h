@@ -1,0 +1,92 @@
+// Device helpers shared by the engine's kernels (gfx950, wave64).
+#pragma once
+
+#include "engine.h"
+
+namespace care_engine {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ u64 murmur64(u64 x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+// base i of a 2-bit packed row (first base in bits 31..30)
+__device__ __forceinline__ u32 base_at(const u32* enc, int i) {
+    return (enc[i >> 4] >> (30 - 2 * (i & 15))) & 3u;
+}
+
+// base i of a read as it appears in the MSA: forward, or reverse complement
+__device__ __forceinline__ u32 oriented_base(const u32* enc, int len, int i, bool rc) {
+    return rc ? 3u - base_at(enc, len - 1 - i) : base_at(enc, i);
+}
+
+// character j of a binned quality row (8, 2 or 1 bits per base)
+__device__ __forceinline__ char quality_char(const u32* enc, int j, int bits) {
+    if (bits == 8) {
+        const u32 w = enc[j >> 2];
+        return char((w >> (8 * (j & 3))) & 0xFFu);  // little-endian byte order
+    }
+    if (bits == 2) {
+        const u32 v = (enc[j >> 4] >> (30 - 2 * (j & 15))) & 3u;
+        return "$(19"[v];
+    }
+    const u32 v = (enc[j >> 5] >> (31 - (j & 31))) & 1u;
+    return v ? '5' : '(';
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ u32 wave_min_u32(u32 v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const u32 w = __shfl_xor(v, o, kWave);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Exclusive prefix count of `flag` over the whole block, in thread order.
+// `scratch` must hold blockDim.x/64 + 1 ints.  Returns the prefix for this
+// thread; *total receives the block-wide count.  All threads must call.
+__device__ __forceinline__ int block_exclusive_count(bool flag, int* scratch, int* total) {
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    const int nwaves = (blockDim.x + kWave - 1) / kWave;
+    const unsigned long long bal = __ballot(flag);
+    const unsigned long long below = lane == 0 ? 0ull : (bal & (~0ull >> (64 - lane)));
+    const int inWave = __popcll(below);
+    __syncthreads();
+    if (lane == 0) scratch[wave] = __popcll(bal);
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < nwaves; w++) {
+        const int c = scratch[w];
+        if (w < wave) before += c;
+        all += c;
+    }
+    *total = all;
+    return before + inWave;
+}
+
+__device__ __forceinline__ unsigned next_pow2(unsigned v) {
+    if (v <= 1) return 1;
+    return 1u << (32 - __clz(v - 1));
+}
+
+}  // namespace care_engine

@@ -1,0 +1,724 @@
+// Host side of the MI355X CARE corrector: the `care-gpu` command line.
+//
+// STEP1  parse inputs, pack reads (2-bit, N rule, binned qualities), upload
+//        to HBM and build the minhash tables on the device;
+// STEP2  correct all reads in batches on the device, turning every batch's
+//        corrected sequences into edit lists on the host;
+// STEP3  combine the results per read and write the ordered output.
+//
+// Mirrors src/gpu/dispatch_care_correct_gpu.cu:522-929 (three timed steps),
+// src/options.cpp:705-895 (options), include/chunkedreadstorageconstruction.hpp
+// (read packing), src/correctionresultoutput.cpp:31-445,730-1023 (merge and
+// combine) and src/readlibraryio.cpp:30-130 (output format).
+
+#include "../../include/care_amd.h"
+#include "../../include/care_host.h"
+
+#include <zlib.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <sys/stat.h>
+#include <thread>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_hostError;
+
+using u8 = std::uint8_t;
+using u32 = std::uint32_t;
+using u64 = std::uint64_t;
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+// Input: FASTA/FASTQ, plain or gzip, parsed like kseq (header = the whole
+// line after '@' or '>', multi-line sequences, '+' line skipped).
+// ---------------------------------------------------------------------------
+class ByteStream {
+public:
+    explicit ByteStream(const std::string& path) {
+        f_ = gzopen(path.c_str(), "rb");
+        if (!f_) throw std::runtime_error("cannot open input file " + path);
+        gzbuffer(f_, 1 << 20);
+        buf_.resize(1 << 22);
+    }
+    ~ByteStream() {
+        if (f_) gzclose(f_);
+    }
+    int get() {
+        if (pos_ >= end_) {
+            if (eof_) return -1;
+            const int r = gzread(f_, buf_.data(), unsigned(buf_.size()));
+            if (r <= 0) {
+                eof_ = true;
+                return -1;
+            }
+            pos_ = 0;
+            end_ = size_t(r);
+        }
+        return (unsigned char)buf_[pos_++];
+    }
+    // appends bytes up to '\n' (dropped); false if nothing was read
+    bool line(std::string& s) {
+        bool any = false;
+        for (;;) {
+            if (pos_ >= end_) {
+                if (get() < 0) break;
+                pos_--;
+            }
+            const char* b = buf_.data() + pos_;
+            const char* e = buf_.data() + end_;
+            const char* nl = static_cast<const char*>(std::memchr(b, '\n', size_t(e - b)));
+            any = true;
+            if (nl) {
+                s.append(b, nl);
+                pos_ += size_t(nl - b) + 1;
+                break;
+            }
+            s.append(b, e);
+            pos_ = end_;
+        }
+        if (!s.empty() && s.back() == '\r') s.pop_back();
+        return any;
+    }
+
+private:
+    gzFile f_ = nullptr;
+    std::vector<char> buf_;
+    size_t pos_ = 0, end_ = 0;
+    bool eof_ = false;
+};
+
+struct RecordReader {
+    ByteStream in;
+    int pending = 0;
+    bool fastq = false;
+    explicit RecordReader(const std::string& p) : in(p) {}
+    bool next(std::string& header, std::string& seq, std::string& qual) {
+        int c = 0;
+        if (pending == 0) {
+            while ((c = in.get()) >= 0 && c != '>' && c != '@') {}
+            if (c < 0) return false;
+            pending = c;
+        }
+        header.clear();
+        seq.clear();
+        qual.clear();
+        if (!in.line(header)) return false;
+        while ((c = in.get()) >= 0 && c != '>' && c != '+' && c != '@') {
+            if (c == '\n') continue;
+            seq.push_back(char(c));
+            in.line(seq);
+        }
+        pending = (c == '>' || c == '@') ? c : 0;
+        if (c != '+') return true;
+        fastq = true;
+        while ((c = in.get()) >= 0 && c != '\n') {}
+        while (qual.size() < seq.size())
+            if (!in.line(qual)) break;
+        pending = 0;
+        if (qual.size() != seq.size()) throw std::runtime_error("quality string length differs from sequence length");
+        return true;
+    }
+};
+
+// All reads of a library in read-id order, stored in arenas.
+struct Library {
+    std::vector<char> headers, seqs, quals;
+    std::vector<u64> hoff, soff;  // n + 1 offsets (quals share soff)
+    std::vector<u32> fileOf;      // first read id of every input file, for the N rule batches
+    bool fastq = true;
+    u64 size() const { return soff.empty() ? 0 : soff.size() - 1; }
+    void add(const std::string& h, const std::string& s, const std::string& q, bool keepQual) {
+        if (hoff.empty()) {
+            hoff.push_back(0);
+            soff.push_back(0);
+        }
+        headers.insert(headers.end(), h.begin(), h.end());
+        hoff.push_back(headers.size());
+        seqs.insert(seqs.end(), s.begin(), s.end());
+        if (keepQual) quals.insert(quals.end(), q.begin(), q.end());
+        soff.push_back(seqs.size());
+    }
+    const char* seq(u64 i) const { return seqs.data() + soff[i]; }
+    int len(u64 i) const { return int(soff[i + 1] - soff[i]); }
+};
+
+Library readLibrary(const std::vector<std::string>& files, bool pairedEnd, bool keepQual) {
+    Library lib;
+    std::string h1, s1, q1, h2, s2, q2;
+    if (pairedEnd && files.size() == 2) {
+        // include/readlibraryio.hpp:130-190: mates of two files interleaved
+        RecordReader a(files[0]), b(files[1]);
+        lib.fileOf.push_back(0);
+        while (a.next(h1, s1, q1)) {
+            if (!b.next(h2, s2, q2)) break;
+            lib.add(h1, s1, q1, keepQual);
+            lib.add(h2, s2, q2, keepQual);
+        }
+        lib.fastq = a.fastq;
+    } else {
+        for (const auto& f : files) {
+            RecordReader a(f);
+            lib.fileOf.push_back(u32(lib.size()));
+            while (a.next(h1, s1, q1)) lib.add(h1, s1, q1, keepQual);
+            lib.fastq = a.fastq;
+        }
+    }
+    if (lib.hoff.empty()) {
+        lib.hoff.push_back(0);
+        lib.soff.push_back(0);
+    }
+    return lib;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Packed read store (include/chunkedreadstorageconstruction.hpp:67-92,216-300)
+// ---------------------------------------------------------------------------
+struct care_readset {
+    u64 n = 0;
+    int maxLen = 0;
+    int pitch = 0;
+    int qbits = 0;
+    int qpitch = 0;
+    std::vector<u32> seqs;
+    std::vector<int32_t> lens;
+    std::vector<u8> ambig;
+    std::vector<u32> quals;
+};
+
+namespace {
+
+inline u32 code2(char c) {
+    switch (c) {
+        case 'C': return 1;
+        case 'G': return 2;
+        case 'T': return 3;
+        default: return 0;
+    }
+}
+
+void encodeQualityRow(u32* out, const char* q, int len, int bits) {
+    if (bits == 8) {
+        std::memcpy(out, q, size_t(len));
+        return;
+    }
+    const int per = 32 / bits;
+    const int words = (bits * len + 31) / 32;
+    for (int w = 0; w < words; w++) {
+        u32 data = 0;
+        const int e = std::min(len, (w + 1) * per);
+        for (int p = w * per; p < e; p++) {
+            const int Q = int(q[p]) - 33;
+            u32 bin;
+            if (bits == 1) bin = Q <= 13 ? 0 : 1;
+            else bin = Q <= 4 ? 0 : Q <= 13 ? 1 : Q <= 20 ? 2 : 3;
+            data = (data << bits) | bin;
+        }
+        if (w == words - 1) data <<= ((w + 1) * per - len) * bits;
+        out[w] = data;
+    }
+}
+
+// Packs every read; non-ACGT characters become A,C,G,T in turn with a counter
+// that restarts at every block of 65536 reads counted from each file start.
+std::unique_ptr<care_readset> packLibrary(const Library& lib, int qualBits, int threads) {
+    auto rs = std::make_unique<care_readset>();
+    rs->n = lib.size();
+    int maxLen = 0;
+    for (u64 i = 0; i < rs->n; i++) maxLen = std::max(maxLen, lib.len(i));
+    rs->maxLen = maxLen;
+    rs->pitch = (maxLen + 15) / 16;
+    rs->qbits = qualBits;
+    rs->qpitch = qualBits ? (qualBits * maxLen + 31) / 32 : 0;
+    rs->seqs.assign(rs->n * rs->pitch, 0);
+    rs->lens.resize(rs->n);
+    rs->ambig.assign(rs->n, 0);
+    if (qualBits) rs->quals.assign(rs->n * u64(rs->qpitch), 0);
+    // batches of the N rule
+    std::vector<std::pair<u64, u64>> batches;
+    std::vector<u64> starts(lib.fileOf.begin(), lib.fileOf.end());
+    starts.push_back(rs->n);
+    for (size_t f = 0; f + 1 < starts.size(); f++)
+        for (u64 b = starts[f]; b < starts[f + 1]; b += 65536) batches.emplace_back(b, std::min(starts[f + 1], b + 65536));
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        std::vector<char> tmp;
+        for (;;) {
+            const size_t bi = next.fetch_add(1);
+            if (bi >= batches.size()) break;
+            int ncount = 0;
+            for (u64 i = batches[bi].first; i < batches[bi].second; i++) {
+                const int len = lib.len(i);
+                const char* s = lib.seq(i);
+                tmp.assign(s, s + len);
+                bool amb = false;
+                for (auto& c : tmp) {
+                    if (c == 'a') c = 'A';
+                    else if (c == 'c') c = 'C';
+                    else if (c == 'g') c = 'G';
+                    else if (c == 't') c = 'T';
+                    else if (c != 'A' && c != 'C' && c != 'G' && c != 'T') {
+                        c = "ACGT"[ncount];
+                        ncount = (ncount + 1) & 3;
+                        amb = true;
+                    }
+                }
+                rs->ambig[i] = amb;
+                rs->lens[i] = len;
+                u32* row = rs->seqs.data() + i * rs->pitch;
+                for (int p = 0; p < len; p++) row[p >> 4] |= code2(tmp[p]) << (30 - 2 * (p & 15));
+                if (qualBits) encodeQualityRow(rs->quals.data() + i * rs->qpitch, lib.quals.data() + lib.soff[i], len, qualBits);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::max(1, threads); t++) pool.emplace_back(work);
+    for (auto& t : pool) t.join();
+    return rs;
+}
+
+// ---------------------------------------------------------------------------
+// Options (src/options.cpp:705-895; defaults include/options.hpp:36-98)
+// ---------------------------------------------------------------------------
+struct HostOptions {
+    care_options eng{};
+    std::vector<std::string> inputs, outputs;
+    std::string outdir;
+    int qualityBits = 8;
+    int threads = 1;
+    u64 fixedNumberOfReads = 0;
+    bool labels = false;
+    bool gzoutput = false;
+    bool showProgress = false;
+    std::vector<int> gpus;
+    int batchsize = 0;
+    std::string forestAnchor, forestCands;
+    bool haveCoverage = false, havePairmode = false;
+};
+
+std::vector<int> parseIntList(const std::string& s) {
+    std::vector<int> v;
+    size_t p = 0;
+    while (p < s.size()) {
+        size_t e = s.find(',', p);
+        if (e == std::string::npos) e = s.size();
+        v.push_back(std::stoi(s.substr(p, e - p)));
+        p = e + 1;
+    }
+    return v;
+}
+
+HostOptions parseOptions(int argc, char** argv) {
+    HostOptions o;
+    care_options_default(&o.eng);
+    auto boolValue = [](const std::string& v) { return !(v == "false" || v == "0" || v == "no"); };
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        std::string inlineVal;
+        bool hasInline = false;
+        if (a.rfind("--", 0) == 0) {
+            const size_t eq = a.find('=');
+            if (eq != std::string::npos) {
+                inlineVal = a.substr(eq + 1);
+                a = a.substr(0, eq);
+                hasInline = true;
+            }
+        }
+        auto value = [&]() -> std::string {
+            if (hasInline) return inlineVal;
+            if (i + 1 >= argc) throw std::runtime_error("option " + a + " needs a value");
+            return argv[++i];
+        };
+        auto flag = [&]() -> bool { return hasInline ? boolValue(inlineVal) : true; };
+        if (a == "-i" || a == "--inputfiles") o.inputs.push_back(value());
+        else if (a == "-o" || a == "--outputfilenames") o.outputs.push_back(value());
+        else if (a == "-d" || a == "--outdir") o.outdir = value();
+        else if (a == "-c" || a == "--coverage") { o.eng.coverage = std::stof(value()); o.haveCoverage = true; }
+        else if (a == "--pairmode") {
+            const std::string m = value();
+            if (m == "PE" || m == "pe") o.eng.paired_end = 1;
+            else if (m == "SE" || m == "se") o.eng.paired_end = 0;
+            else throw std::runtime_error("invalid pairmode " + m);
+            o.havePairmode = true;
+        } else if (a == "-g" || a == "--gpu") o.gpus = parseIntList(value());
+        else if (a == "-k" || a == "--kmerlength") o.eng.kmer_length = std::stoi(value());
+        else if (a == "-h" || a == "--hashmaps") o.eng.num_hash_maps = std::stoi(value());
+        else if (a == "-t" || a == "--threads") o.threads = std::stoi(value());
+        else if (a == "-q" || a == "--useQualityScores") o.eng.use_quality = flag();
+        else if (a == "--qualityScoreBits") o.qualityBits = std::stoi(value());
+        else if (a == "--excludeAmbiguous") o.eng.exclude_ambiguous = flag();
+        else if (a == "--maxmismatchratio") o.eng.max_mismatch_ratio = std::stof(value());
+        else if (a == "--minalignmentoverlap") o.eng.min_overlap = std::stoi(value());
+        else if (a == "--minalignmentoverlapratio") o.eng.min_overlap_ratio = std::stof(value());
+        else if (a == "--errorfactortuning") o.eng.error_factor = std::stof(value());
+        else if (a == "--coveragefactortuning") o.eng.coverage_factor = std::stof(value());
+        else if (a == "--candidateCorrection") o.eng.correct_candidates = flag();
+        else if (a == "--candidateCorrectionNewColumns") o.eng.new_columns_to_correct = std::stoi(value());
+        else if (a == "--correctionQualityLabels") o.labels = flag();
+        else if (a == "--correctionType") o.eng.correction_type = std::stoi(value());
+        else if (a == "--correctionTypeCands") o.eng.correction_type_cands = std::stoi(value());
+        else if (a == "--ml-forestfile") o.forestAnchor = value();
+        else if (a == "--ml-cands-forestfile") o.forestCands = value();
+        else if (a == "--thresholdAnchor") o.eng.threshold_anchor = std::stof(value());
+        else if (a == "--thresholdCands") o.eng.threshold_cands = std::stof(value());
+        else if (a == "--pairedFilterThreshold") o.eng.paired_filter_threshold = std::stof(value());
+        else if (a == "--fixedNumberOfReads") o.fixedNumberOfReads = std::stoull(value());
+        else if (a == "--batchsize") o.batchsize = std::stoi(value());
+        else if (a == "--gzoutput") o.gzoutput = flag();
+        else if (a == "-p" || a == "--showProgress") o.showProgress = flag();
+        else if (a == "--enforceHashmapCount") (void)flag();
+        else if (a == "-m" || a == "--memTotal" || a == "--memHashtables" || a == "--tempdir" || a == "--hashloadfactor" ||
+                 a == "--warpcore" || a == "--replicateGpuData" || a == "--save-preprocessedreads-to" ||
+                 a == "--load-preprocessedreads-from" || a == "--save-hashtables-to" || a == "--load-hashtables-from" ||
+                 a == "--maxForestTreesAnchor" || a == "--maxForestTreesCands" || a == "--singlehash" ||
+                 a == "--samplingRateAnchor" || a == "--samplingRateCands")
+            (void)value();
+        else throw std::runtime_error("unknown option " + a);
+    }
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// Correction results (include/correctedsequence.hpp:22-146 semantics)
+// ---------------------------------------------------------------------------
+struct ResultStore {
+    // one entry per stored result
+    std::vector<u32> readId;
+    std::vector<u8> flags;      // bit0 anchor, bit1 hq, bit2 useEdits
+    std::vector<u64> dataOff;   // offset into data (edits: pos u16 + base; or full sequence)
+    std::vector<u32> dataLen;   // number of edits or sequence length
+    std::vector<char> data;
+
+    void add(u32 id, bool anchor, bool hq, const char* corrected, const char* original, int len, bool ambiguous) {
+        // include/corrector.hpp:1645-1668 and 1717-1757
+        u8 f = u8((anchor ? 1 : 0) | (hq ? 2 : 0));
+        const u64 off = data.size();
+        u32 cnt = 0;
+        bool useEdits = false;
+        if (!ambiguous) {
+            const int maxEdits = len / 7;
+            int edits = 0;
+            for (int i = 0; i < len && edits <= maxEdits; i++) {
+                if (corrected[i] != original[i]) {
+                    data.push_back(char(i & 0xFF));
+                    data.push_back(char((i >> 8) & 0xFF));
+                    data.push_back(corrected[i]);
+                    edits++;
+                }
+            }
+            useEdits = edits <= maxEdits;
+            if (useEdits) cnt = u32(edits);
+            else data.resize(off);
+        }
+        if (useEdits) {
+            // HQ anchors without edits are not stored (src/correct_cpu.cpp:262-266)
+            if (anchor && hq && cnt == 0) return;
+            f |= 4;
+        } else {
+            data.insert(data.end(), corrected, corrected + len);
+            cnt = u32(len);
+        }
+        readId.push_back(id);
+        flags.push_back(f);
+        dataOff.push_back(off);
+        dataLen.push_back(cnt);
+    }
+};
+
+// src/correctionresultoutput.cpp:730-1023 (combineMultipleCorrectionResults1_rawtcs2);
+// returns label 0 uncorrected, 1 lq anchor only, 2 lq with candidates, 3 hq
+int combineRead(const ResultStore& rsx, const std::vector<u64>& entries, std::string& seq) {
+    if (entries.empty()) return 0;
+    long anchorIdx = -1;
+    for (u64 e : entries)
+        if (rsx.flags[e] & 1) {
+            anchorIdx = long(e);
+            break;
+        }
+    if (anchorIdx < 0) return 0;
+    auto apply = [&](u64 e, std::string& s) {
+        if (rsx.flags[e] & 4) {
+            const char* d = rsx.data.data() + rsx.dataOff[e];
+            for (u32 k = 0; k < rsx.dataLen[e]; k++) {
+                const int pos = int((unsigned char)d[3 * k]) | (int((unsigned char)d[3 * k + 1]) << 8);
+                s[size_t(pos)] = d[3 * k + 2];
+            }
+        } else {
+            s.assign(rsx.data.data() + rsx.dataOff[e], rsx.dataLen[e]);
+        }
+    };
+    const u64 a = u64(anchorIdx);
+    if (rsx.flags[a] & 2) {
+        apply(a, seq);
+        return 3;
+    }
+    if (entries.size() >= 3) {
+        const u64 f0 = entries[0];
+        std::string first = seq;
+        apply(f0, first);
+        for (size_t i = 1; i < entries.size(); i++) {
+            const u64 e = entries[i];
+            bool eq;
+            const bool e0 = rsx.flags[f0] & 4, e1 = rsx.flags[e] & 4;
+            if (e0 && e1) {
+                eq = rsx.dataLen[f0] == rsx.dataLen[e] &&
+                     std::memcmp(rsx.data.data() + rsx.dataOff[f0], rsx.data.data() + rsx.dataOff[e], 3 * size_t(rsx.dataLen[e])) == 0;
+            } else if (!e0 && !e1) {
+                eq = rsx.dataLen[f0] == rsx.dataLen[e] &&
+                     std::memcmp(rsx.data.data() + rsx.dataOff[f0], rsx.data.data() + rsx.dataOff[e], rsx.dataLen[e]) == 0;
+            } else {
+                std::string other = seq;
+                apply(e, other);
+                eq = other == first;
+            }
+            if (!eq) return 0;
+        }
+        seq.swap(first);
+        return 2;
+    }
+    apply(a, seq);
+    return 1;
+}
+
+struct OutWriter {
+    FILE* f = nullptr;
+    gzFile gz = nullptr;
+    std::string buf;
+    OutWriter(const std::string& path, bool gzip) {
+        if (gzip) {
+            gz = gzopen(path.c_str(), "wb");
+            if (!gz) throw std::runtime_error("cannot open output file " + path);
+        } else {
+            f = std::fopen(path.c_str(), "wb");
+            if (!f) throw std::runtime_error("cannot open output file " + path);
+        }
+    }
+    void flush() {
+        if (buf.empty()) return;
+        if (f) std::fwrite(buf.data(), 1, buf.size(), f);
+        else gzwrite(gz, buf.data(), unsigned(buf.size()));
+        buf.clear();
+    }
+    ~OutWriter() {
+        flush();
+        if (f) std::fclose(f);
+        if (gz) gzclose(gz);
+    }
+};
+
+int runMain(int argc, char** argv) {
+    HostOptions o = parseOptions(argc, argv);
+    if (o.inputs.empty() || o.outputs.empty() || o.outdir.empty() || !o.haveCoverage || !o.havePairmode) {
+        std::fprintf(stderr, "Mandatory options: -i/--inputfiles, -o/--outputfilenames, -d/--outdir, -c/--coverage, --pairmode, -g/--gpu\n");
+        return 0;  // the reference exits with status 0 on invalid options
+    }
+    if (o.gpus.empty()) {
+        std::fprintf(stderr, "No GPU specified (-g). This build runs on one device, e.g. -g 0\n");
+        return 0;
+    }
+    if (o.gpus.size() > 1) std::fprintf(stderr, "Using only GPU %d of the given list (single-device build)\n", o.gpus[0]);
+    if (o.outputs.size() != 1 && o.outputs.size() != o.inputs.size())
+        throw std::runtime_error("number of output files must be 1 or equal to the number of input files");
+    if (o.eng.paired_end && o.inputs.size() > 2) throw std::runtime_error("paired-end input takes one or two files");
+    if (o.eng.use_quality && o.qualityBits != 1 && o.qualityBits != 2 && o.qualityBits != 8)
+        throw std::runtime_error("qualityScoreBits must be 1, 2 or 8");
+    mkdir(o.outdir.c_str(), 0755);
+    const int threads = std::max(1, o.threads);
+
+    // STEP 1
+    std::printf("STEP 1: Database construction\n");
+    const double t0 = now_s();
+    Library lib = readLibrary(o.inputs, o.eng.paired_end != 0, true);
+    if (!lib.fastq) o.eng.use_quality = 0;
+    std::unique_ptr<care_readset> rs = packLibrary(lib, o.eng.use_quality ? o.qualityBits : 0, threads);
+    std::printf("Number of reads: %llu, max length %d\n", (unsigned long long)rs->n, rs->maxLen);
+    care_context* ctx = nullptr;
+    auto check = [&](int rc) {
+        if (rc != CARE_OK) throw std::runtime_error(std::string("engine: ") + care_last_error());
+    };
+    check(care_context_create(o.gpus[0], &o.eng, &ctx));
+    std::unique_ptr<care_context, int (*)(care_context*)> guard(ctx, care_context_destroy);
+    if (o.batchsize > 0) check(care_set_launch_config(ctx, (o.batchsize + 1) & ~1, 0));
+    check(care_readstore_upload(ctx, rs->n, rs->maxLen, rs->seqs.data(), rs->pitch, rs->lens.data(), rs->ambig.data(),
+                                o.eng.use_quality ? rs->quals.data() : nullptr, rs->qpitch, rs->qbits));
+    check(care_minhash_build(ctx));
+    const double t1 = now_s();
+    std::printf("# elapsed time (STEP1): %.6fs\n", t1 - t0);
+
+    // STEP 2
+    std::printf("STEP 2: Error correction\n");
+    const u64 numAnchors = o.fixedNumberOfReads ? std::min<u64>(o.fixedNumberOfReads, rs->n) : rs->n;
+    const int pitch = (rs->maxLen + 15) / 16 * 16;
+    const int chunk = 1 << 18;
+    std::vector<u8> status(chunk);
+    std::vector<char> seqs(size_t(chunk) * pitch);
+    const int candCap = o.eng.correct_candidates ? chunk * 64 : 0;
+    std::vector<u32> candIds(std::max(1, candCap));
+    std::vector<char> candSeqs(size_t(std::max(1, candCap)) * pitch);
+    ResultStore anchors, cands;
+    std::vector<u8> hqAnchor(rs->n, 0);
+    std::string original;
+    auto decode = [&](u64 id, std::string& out) {
+        const int len = rs->lens[id];
+        out.resize(size_t(len));
+        const u32* row = rs->seqs.data() + id * rs->pitch;
+        for (int p = 0; p < len; p++) out[size_t(p)] = "ACGT"[(row[p >> 4] >> (30 - 2 * (p & 15))) & 3u];
+    };
+    for (u64 first = 0; first < numAnchors; first += chunk) {
+        int n = int(std::min<u64>(chunk, numAnchors - first));
+        if (o.eng.paired_end && (n & 1)) n++;  // whole pairs
+        if (first + u64(n) > rs->n) n = int(rs->n - first);
+        care_batch_result r{};
+        r.anchor_status = status.data();
+        r.anchor_sequences = seqs.data();
+        r.sequence_pitch = pitch;
+        r.candidate_read_ids = candIds.data();
+        r.candidate_sequences = candSeqs.data();
+        r.candidate_capacity = candCap;
+        check(care_correct_batch(ctx, first, n, &r));
+        for (int i = 0; i < n; i++) {
+            if (!(status[size_t(i)] & 1)) continue;
+            const u64 id = first + u64(i);
+            const bool hq = (status[size_t(i)] & 2) != 0;
+            if (hq) hqAnchor[id] = 1;
+            decode(id, original);
+            anchors.add(u32(id), true, hq, seqs.data() + size_t(i) * pitch, original.data(), rs->lens[id], rs->ambig[id] != 0);
+        }
+        for (int i = 0; i < r.num_candidates; i++) {
+            const u32 id = candIds[size_t(i)];
+            decode(id, original);
+            cands.add(id, false, false, candSeqs.data() + size_t(i) * pitch, original.data(), rs->lens[id], rs->ambig[id] != 0);
+        }
+    }
+    const double t2 = now_s();
+    std::printf("# elapsed time (STEP2): %.6fs\n", t2 - t1);
+    std::printf("Correction throughput : ~%.1f reads/second.\n", double(numAnchors) / (t2 - t1));
+
+    // STEP 3
+    std::printf("STEP 3: Constructing output file(s)\n");
+    // per-read entry lists; candidate results of reads corrected as HQ
+    // anchors are dropped (include/corrector.hpp:1693-1697)
+    std::vector<u32> count(rs->n + 1, 0);
+    for (size_t e = 0; e < anchors.readId.size(); e++) count[anchors.readId[e]]++;
+    for (size_t e = 0; e < cands.readId.size(); e++)
+        if (!hqAnchor[cands.readId[e]]) count[cands.readId[e]]++;
+    std::vector<u64> start(rs->n + 1, 0);
+    for (u64 i = 0; i < rs->n; i++) start[i + 1] = start[i] + count[i];
+    // merged store: anchors first, then candidates (offsets into one index space)
+    const u64 na = anchors.readId.size();
+    std::vector<u64> entry(start[rs->n]);
+    std::vector<u64> fill(start.begin(), start.end() - 1);
+    for (u64 e = 0; e < na; e++) entry[fill[anchors.readId[e]]++] = e;
+    for (u64 e = 0; e < cands.readId.size(); e++)
+        if (!hqAnchor[cands.readId[e]]) entry[fill[cands.readId[e]]++] = na + e;
+    ResultStore all;
+    all.readId = anchors.readId;
+    all.readId.insert(all.readId.end(), cands.readId.begin(), cands.readId.end());
+    all.flags = anchors.flags;
+    all.flags.insert(all.flags.end(), cands.flags.begin(), cands.flags.end());
+    all.dataLen = anchors.dataLen;
+    all.dataLen.insert(all.dataLen.end(), cands.dataLen.begin(), cands.dataLen.end());
+    all.dataOff = anchors.dataOff;
+    for (u64 off : cands.dataOff) all.dataOff.push_back(off + anchors.data.size());
+    all.data.swap(anchors.data);
+    all.data.insert(all.data.end(), cands.data.begin(), cands.data.end());
+    long long lastWithResult = -1;
+    for (u64 i = 0; i < rs->n; i++)
+        if (count[i]) lastWithResult = (long long)i;
+
+    const bool perFile = o.outputs.size() > 1;
+    std::vector<std::unique_ptr<OutWriter>> writers;
+    for (const auto& name : o.outputs) writers.push_back(std::make_unique<OutWriter>(o.outdir + "/" + name, o.gzoutput));
+    std::vector<u64> fileEnd;
+    for (size_t f = 1; f < lib.fileOf.size(); f++) fileEnd.push_back(lib.fileOf[f]);
+    fileEnd.push_back(rs->n);
+    size_t fileIdx = 0;
+    std::string seq;
+    std::vector<u64> entries;
+    for (u64 i = 0; i < rs->n; i++) {
+        while (perFile && fileIdx + 1 < fileEnd.size() && i >= fileEnd[fileIdx]) fileIdx++;
+        OutWriter& w = *writers[perFile ? fileIdx : 0];
+        seq.assign(lib.seq(i), size_t(lib.len(i)));
+        entries.assign(entry.begin() + long(start[i]), entry.begin() + long(start[i + 1]));
+        const int label = combineRead(all, entries, seq);
+        w.buf += lib.fastq ? '@' : '>';
+        w.buf.append(lib.headers.data() + lib.hoff[i], lib.hoff[i + 1] - lib.hoff[i]);
+        if (o.labels && (long long)i <= lastWithResult) {
+            static const char* lab[4] = {" care:q=0", " care:q=1", " care:q=2", " care:q=3"};
+            w.buf += lab[label];
+        }
+        w.buf += '\n';
+        w.buf += seq;
+        w.buf += '\n';
+        if (lib.fastq) {
+            w.buf += "+\n";
+            w.buf.append(lib.quals.data() + lib.soff[i], size_t(lib.len(i)));
+            w.buf += '\n';
+        }
+        if (w.buf.size() > (size_t(1) << 22)) w.flush();
+    }
+    writers.clear();
+    const double t3 = now_s();
+    std::printf("# elapsed time (STEP3): %.6fs\n", t3 - t2);
+    std::printf("Construction of output file(s) finished.\n");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int care_host_main(int argc, char** argv) {
+    try {
+        return runMain(argc, argv);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "care-gpu: %s\n", e.what());
+        return 1;
+    }
+}
+
+const char* care_host_last_error(void) { return g_hostError.c_str(); }
+
+care_readset* care_readset_load(const char* const* files, int num_files, int paired_end, int qual_bits) {
+    try {
+        if (!files || num_files <= 0) throw std::runtime_error("no input files");
+        if (qual_bits != 0 && qual_bits != 1 && qual_bits != 2 && qual_bits != 8)
+            throw std::runtime_error("qual_bits must be 0, 1, 2 or 8");
+        std::vector<std::string> f(files, files + num_files);
+        Library lib = readLibrary(f, paired_end != 0, qual_bits != 0);
+        if (!lib.fastq) qual_bits = 0;
+        const unsigned hc = std::thread::hardware_concurrency();
+        return packLibrary(lib, qual_bits, int(hc ? std::min(hc, 16u) : 4u)).release();
+    } catch (const std::exception& e) {
+        g_hostError = e.what();
+        return nullptr;
+    }
+}
+
+void care_readset_free(care_readset* rs) { delete rs; }
+uint64_t care_readset_size(const care_readset* rs) { return rs ? rs->n : 0; }
+int care_readset_max_length(const care_readset* rs) { return rs ? rs->maxLen : 0; }
+int care_readset_seq_pitch(const care_readset* rs) { return rs ? rs->pitch : 0; }
+int care_readset_qual_pitch(const care_readset* rs) { return rs ? rs->qpitch : 0; }
+const uint32_t* care_readset_seqs(const care_readset* rs) { return rs ? rs->seqs.data() : nullptr; }
+const int32_t* care_readset_lengths(const care_readset* rs) { return rs ? rs->lens.data() : nullptr; }
+const uint8_t* care_readset_ambiguous(const care_readset* rs) { return rs ? rs->ambig.data() : nullptr; }
+const uint32_t* care_readset_quals(const care_readset* rs) { return rs && !rs->quals.empty() ? rs->quals.data() : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+/*
+ * care_amd.h - C-ABI of the MI355X correction engine for CARE.
+ *
+ * The engine replaces the device side of `care-gpu ... -g 0`: read storage in
+ * HBM, the minhasher (signatures, tables, candidate queries) and the
+ * correction pipeline (alignment, MSA, refinement, anchor and candidate
+ * correction).  Every entry point takes plain pointers and sizes, returns
+ * CARE_OK (0) on success or a CARE_ERR_* code, never throws, and records a
+ * message retrievable with care_last_error().  Host buffers passed in are
+ * only read during the call; host output buffers are filled before the call
+ * returns.  A context is bound to one device and is not safe for concurrent
+ * use from several threads; distinct contexts are independent.
+ */
+#ifndef CARE_AMD_H
+#define CARE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CARE_OK 0
+#define CARE_ERR_ARGUMENT 1  /* invalid argument or size out of range */
+#define CARE_ERR_DEVICE 2    /* HIP runtime error (allocation, launch, copy) */
+#define CARE_ERR_STATE 3     /* call made in the wrong order */
+#define CARE_ERR_CAPACITY 4  /* an output buffer was too small */
+
+/* Correction parameters.  Field meaning and defaults follow the reference's
+ * ProgramOptions (include/options.hpp:36-98); care_options_default() fills
+ * those defaults. */
+typedef struct care_options {
+    int kmer_length;             /* -k, 1..31 */
+    int num_hash_maps;           /* -h, 1..64 */
+    float coverage;              /* -c, estimated coverage */
+    int paired_end;              /* --pairmode PE: mates are reads 2i, 2i+1 */
+    int use_quality;             /* -q */
+    int exclude_ambiguous;       /* --excludeAmbiguous */
+    int correct_candidates;      /* --candidateCorrection */
+    int new_columns_to_correct;  /* --candidateCorrectionNewColumns */
+    int min_overlap;             /* --minalignmentoverlap */
+    float max_mismatch_ratio;    /* --maxmismatchratio */
+    float min_overlap_ratio;     /* --minalignmentoverlapratio */
+    float error_factor;          /* --errorfactortuning (estimatedErrorrate) */
+    float coverage_factor;       /* --coveragefactortuning (m_coverage) */
+    float paired_filter_threshold; /* --pairedFilterThreshold */
+    int correction_type;         /* --correctionType: 0 classic, 1 forest, 3 mlp */
+    int correction_type_cands;   /* --correctionTypeCands: 0 classic, 1 forest, 3 mlp */
+    float threshold_anchor;      /* --thresholdAnchor */
+    float threshold_cands;       /* --thresholdCands */
+} care_options;
+
+typedef struct care_context care_context;
+
+void care_options_default(care_options* opt);
+
+/* Thread-local text of the last error reported by this library. */
+const char* care_last_error(void);
+
+/* Creates a context on HIP device `device`.
+ * Replaces the per-device setup of performCorrection
+ * (src/gpu/dispatch_care_correct_gpu.cu:551-602). */
+int care_context_create(int device, const care_options* opt, care_context** out);
+int care_context_destroy(care_context* ctx);
+
+/* Uploads the packed read store to HBM.
+ *   seqs      : num_reads x seq_pitch u32, 2-bit A0 C1 G2 T3, 16 bases per
+ *               word, first base in bits 31..30 (include/sequencehelpers.hpp:154-187)
+ *   lengths   : num_reads read lengths (1..max_length)
+ *   ambiguous : num_reads flags, 1 if the input read held a non-ACGT base
+ *   quals     : num_reads x qual_pitch u32 binned qualities (1, 2 or 8 bits
+ *               per base, include/qualityscorecompression.hpp:28-146), or NULL
+ * Replaces MultiGpuReadStorage construction / rebuild
+ * (include/gpu/multigpureadstorage.cuh, src/gpu/dispatch_care_correct_gpu.cu:773-806). */
+int care_readstore_upload(care_context* ctx, uint64_t num_reads, int max_length, const uint32_t* seqs,
+                          int seq_pitch, const int32_t* lengths, const uint8_t* ambiguous, const uint32_t* quals,
+                          int qual_pitch, int qual_bits);
+
+/* Gathers rows of the device read store into host buffers (sequences with
+ * out_pitch words per row, lengths).  Replaces GpuReadStorage::gatherSequences /
+ * gatherSequenceLengths (include/gpu/gpureadstorage.cuh:22-126). */
+int care_readstore_gather(care_context* ctx, const uint32_t* ids, int n, uint32_t* out_seqs, int out_pitch,
+                          int32_t* out_lengths);
+
+/* Builds all minhash tables on the device from the uploaded read store.
+ * Replaces constructGpuMinhasherFromGpuReadStorage / GpuMinhasher::insert +
+ * compact (src/gpu/gpuminhasherconstruction.cu:37-253,
+ * include/gpu/gpuhashtable.cuh:743). */
+int care_minhash_build(care_context* ctx);
+
+/* Minhash signatures of reads [first, first+n): n x num_hash_maps keys.
+ * Replaces minhashSignatures3264Kernel (include/gpu/gpusequencehasher.cuh:116). */
+int care_minhash_signatures(care_context* ctx, uint64_t first, int n, uint64_t* out);
+
+/* Candidate ids of anchors [first, first+n): for every anchor the sorted,
+ * de-duplicated table hits without the anchor itself (and without ambiguous
+ * reads under exclude_ambiguous).  offsets has n+1 entries.  Returns
+ * CARE_ERR_CAPACITY if more than ids_capacity ids result.
+ * Replaces determineNumValues + retrieveValues + keepDistinctAndNotMatching
+ * (include/gpu/gpuminhasher.cuh:20-110, include/gpu/minhashqueryfilter.cuh:21-100). */
+int care_minhash_query(care_context* ctx, uint64_t first, int n, uint32_t* ids, int32_t* offsets,
+                       uint64_t ids_capacity);
+
+/* Results of one correction batch, filled by care_correct_batch. */
+typedef struct care_batch_result {
+    /* per anchor: bit0 = corrected, bit1 = high-quality MSA */
+    uint8_t* anchor_status;      /* [num_anchors] */
+    char* anchor_sequences;      /* [num_anchors * sequence_pitch] corrected bases */
+    int sequence_pitch;          /* bytes per row, >= max_length */
+    /* candidate corrections (only with correct_candidates) */
+    uint32_t* candidate_read_ids;  /* [candidate_capacity] */
+    char* candidate_sequences;     /* [candidate_capacity * sequence_pitch], forward strand */
+    int candidate_capacity;
+    int num_candidates;            /* out */
+} care_batch_result;
+
+/* Corrects anchors [first, first+n) (n even for paired-end, first even).
+ * Replaces GpuErrorCorrector::correct (include/gpu/gpucorrector.cuh:1060)
+ * with its kernels: alignment, filters, MSA build + refinement, anchor and
+ * candidate correction (src/gpu/{alignment,msa,anchorcorrection,
+ * candidatecorrection,correction}kernels.cu). */
+int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* result);
+
+/* Number of anchors per internal launch and launch shape chosen by the
+ * autotuner (0 = automatic). */
+int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threads);
+
+/* Forest classifier for correction_type / correction_type_cands 1, in the
+ * reference's .rf file format (include/forest.hpp:38-55, ml/care.py:198-282).
+ * which: 0 anchor, 1 candidates.  Replaces GpuForest
+ * (include/gpu/forest_gpu.cuh:133). */
+int care_forest_load(care_context* ctx, int which, const char* path, uint32_t max_trees);
+
+/* Timing and work counters of the last care_correct_batch call. */
+typedef struct care_batch_stats {
+    double query_ms;       /* minhash query kernel */
+    double correct_ms;     /* fused correction kernel */
+    double total_ms;       /* whole call incl. copies */
+    uint64_t raw_hits;     /* table values gathered before de-duplication */
+    uint64_t candidates;   /* candidates after query filtering */
+} care_batch_stats;
+int care_last_batch_stats(care_context* ctx, care_batch_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CARE_AMD_H */

@@ -16,6 +16,7 @@
 
 #include "care_oracle.h"
 
+#include <sys/stat.h>
 #include <zlib.h>
 #include <omp.h>
 
@@ -771,7 +772,9 @@ struct MSA {
     Props properties(float err, float cov, float mcov) const {
         Props p;
         const float avgT = 1.0f - 1.0f * err;
-        const float minT = 1.0f - 3.0f * err;
+        // care-cpu (built -O3 -march=native) evaluates every 1 - 3x threshold
+        // as one fused multiply-add
+        const float minT = std::fma(-3.0f, err, 1.0f);
         const float covT = mcov / 6.0f * cov;
         const int dist = anchorEnd - anchorBegin;
         p.min_support = *std::min_element(support.begin() + anchorBegin, support.begin() + anchorEnd);
@@ -787,7 +790,7 @@ struct MSA {
             const int pct = int(std::ceil(err * 100.0f));
             for (int percent = pct; percent >= 0; percent--) {
                 const float factor = percent / 100.0f;
-                if (fgeq(p.avg_support, 1.0f - 1.0f * factor) && fgeq(p.min_support, 1.0f - 3.0f * factor)) smallest = percent;
+                if (fgeq(p.avg_support, 1.0f - factor) && fgeq(p.min_support, std::fma(-3.0f, factor, 1.0f))) smallest = percent;
             }
             p.isHQ = fgeq(float(p.min_coverage), covT) && fleq(float(smallest), pct * 0.5f);
         }
@@ -797,7 +800,7 @@ struct MSA {
     // src/msa.cpp:379-453
     std::string correctedAnchor(const Props& p, float err, float cov, float mcov) const {
         std::string out(in.anchorLength, 'A');
-        const bool byConsensus = fgeq(p.avg_support, 1.0f - 1.0f * err) && fgeq(p.min_support, 1.0f - 3.0f * err) &&
+        const bool byConsensus = fgeq(p.avg_support, 1.0f - err) && fgeq(p.min_support, std::fma(-3.0f, err, 1.0f)) &&
                                  fgeq(float(p.min_coverage), mcov / 6.0f * cov);
         for (int i = 0; i < in.anchorLength; i++) {
             const int c = anchorBegin + i;
@@ -1416,6 +1419,7 @@ int runMain(int argc, char** argv) {
     const std::string outPath = opt.outputdirectory.empty() ? opt.outputfilenames[0]
                                                             : opt.outputdirectory + "/" + opt.outputfilenames[0];
     const bool fastq = isFastq(opt.inputfiles[0]);
+    if (!opt.outputdirectory.empty()) mkdir(opt.outputdirectory.c_str(), 0755);
     std::ofstream os(outPath);
     if (!os) throw std::runtime_error("cannot open output " + outPath);
     std::string buf;

@@ -1,0 +1,28 @@
+"""Seeded end-to-end cases shared by the golden-fixture script and the tests.
+
+Each case: simulator profile, seed and genome size (care_amd/sim/readsim.cpp)
+and the corrector options (same flags for care-cpu, care-oracle, care-gpu).
+Read counts stay below 65536 per file so that the reference's stale quality
+padding (reused encode buffers, include/chunkedreadstorageconstruction.hpp:
+216-237) cannot occur.
+"""
+
+CASES = {
+    # single-end, no qualities (uniform 100 bp, 1% errors)
+    "se_plain": {"profile": "uniform-se", "seed": 101, "genome": 60000,
+                 "args": ["-c", "30", "--pairmode", "SE", "-k", "20", "-h", "16"]},
+    # paired-end with repeats, 2-bit qualities
+    "pe_quality2": {"profile": "repeat-pe", "seed": 102, "genome": 120000,
+                    "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "2", "-k", "20", "-h", "32"]},
+    # variable length 100-250, Ns, exclude ambiguous, 8-bit qualities
+    "se_mixed_ambig": {"profile": "mixed-se", "seed": 103, "genome": 120000,
+                       "args": ["-c", "50", "--pairmode", "SE", "--excludeAmbiguous", "-q", "-k", "20", "-h", "48"]},
+    # paired-end with candidate correction and quality labels
+    "pe_candidates": {"profile": "large-pe", "seed": 104, "genome": 120000,
+                      "args": ["-c", "40", "--pairmode", "PE", "-q", "--candidateCorrection", "-k", "20", "-h", "48",
+                               "--correctionQualityLabels"]},
+    # variable length without excluding ambiguous reads, 1-bit qualities
+    "se_mixed_q1": {"profile": "mixed-se", "seed": 105, "genome": 80000,
+                    "args": ["-c", "50", "--pairmode", "SE", "-q", "--qualityScoreBits", "1", "-k", "18", "-h", "24",
+                             "--candidateCorrection"]},
+}

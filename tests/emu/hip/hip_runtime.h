@@ -1,0 +1,384 @@
+// CPU emulation of the HIP runtime/device subset used by care_amd/csrc, so
+// the engine's kernel sources can run in this container's tests without a
+// GPU.  Test infrastructure only: it is compiled into a separate library
+// (tests/emu/build_emu.sh) that the CPU tests load; the product is never
+// built against it.
+//
+// Model: a kernel launch runs its blocks one after another; the threads of
+// a block are fibers (ucontext) on the calling OS thread.  __syncthreads()
+// is a barrier over the block, wave (64-lane) operations exchange values
+// through a per-wave buffer between two wave barriers.  A scheduling pass
+// in which no fiber can proceed is reported as a deadlock.  __shared__
+// variables become function statics (one block runs at a time) and dynamic
+// LDS is one buffer per launch.
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <ucontext.h>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(x)
+#define __shared__ static
+#define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(emu::dyn_lds());
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct uint2 {
+    unsigned x, y;
+};
+inline uint2 make_uint2(unsigned a, unsigned b) { return uint2{a, b}; }
+
+typedef int hipError_t;
+typedef void* hipStream_t;
+typedef struct emu_event* hipEvent_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
+enum { hipStreamNonBlocking = 1, hipHostMallocDefault = 0 };
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
+
+namespace emu {
+
+constexpr int kWaveSize = 64;
+
+struct ThreadCtx {
+    dim3 tid, bid, bdim, gdim;
+};
+// All emulated GPU threads are fibers on one OS thread; ctx is the running
+// fiber's context.
+inline ThreadCtx ctx;
+
+struct Fiber {
+    ucontext_t uc;
+    std::vector<char> stack;
+    ThreadCtx c;
+    bool done = false;
+    int waitKind = 0;  // 0 running, 1 block barrier, 2 wave barrier
+    unsigned long waitGen = 0;
+};
+
+struct Wave {
+    unsigned long long vals[kWaveSize];
+    int arrived = 0;
+    int size = 0;
+    unsigned long gen = 0;
+};
+
+struct Sched {
+    std::vector<Fiber> fibers;
+    std::vector<Wave> waves;
+    ucontext_t main;
+    int cur = -1;
+    int blockArrived = 0;
+    unsigned long blockGen = 0;
+    void (*body)(void*) = nullptr;
+    void* arg = nullptr;
+};
+
+inline Sched*& sched() {
+    static Sched* s = nullptr;
+    return s;
+}
+inline std::vector<char>& lds() {
+    static std::vector<char> b;
+    return b;
+}
+inline char* dyn_lds() { return lds().data(); }
+inline int num_cus() { return 2; }
+inline int lane() { return int(ctx.tid.x % kWaveSize); }
+inline Wave& my_wave() { return sched()->waves[ctx.tid.x / kWaveSize]; }
+
+inline void yield_to_scheduler() {
+    Sched* s = sched();
+    Fiber& f = s->fibers[s->cur];
+    swapcontext(&f.uc, &s->main);
+}
+
+inline void block_barrier() {
+    Sched* s = sched();
+    Fiber& f = s->fibers[s->cur];
+    const unsigned long g = s->blockGen;
+    if (++s->blockArrived == int(s->fibers.size())) {
+        s->blockArrived = 0;
+        s->blockGen++;
+        return;
+    }
+    f.waitKind = 1;
+    f.waitGen = g;
+    yield_to_scheduler();
+}
+
+inline void wave_barrier() {
+    Sched* s = sched();
+    Fiber& f = s->fibers[s->cur];
+    Wave& w = my_wave();
+    const unsigned long g = w.gen;
+    if (++w.arrived == w.size) {
+        w.arrived = 0;
+        w.gen++;
+        return;
+    }
+    f.waitKind = 2;
+    f.waitGen = g;
+    yield_to_scheduler();
+}
+
+template <class T>
+inline unsigned long long to_bits(T v) {
+    unsigned long long b = 0;
+    std::memcpy(&b, &v, sizeof(T));
+    return b;
+}
+template <class T>
+inline T from_bits(unsigned long long b) {
+    T v;
+    std::memcpy(&v, &b, sizeof(T));
+    return v;
+}
+template <class T>
+inline T exchange(T v, int src) {
+    Wave& w = my_wave();
+    w.vals[lane()] = to_bits(v);
+    wave_barrier();
+    const T r = from_bits<T>(w.vals[src]);
+    wave_barrier();
+    return r;
+}
+
+inline void fiber_entry() {
+    Sched* s = sched();
+    Fiber& f = s->fibers[s->cur];
+    s->body(s->arg);
+    f.done = true;
+    swapcontext(&f.uc, &s->main);
+}
+
+// Kernels of the engine accept any block size that is a multiple of 64;
+// EMU_BLOCK (default 256) caps the emulated block size.
+inline unsigned block_cap() {
+    const char* e = std::getenv("EMU_BLOCK");
+    const int v = e ? std::atoi(e) : 256;
+    return unsigned(v >= 64 ? v : 64);
+}
+
+// Runs one block to completion; aborts with a report on deadlock.
+inline void run_block(Sched& s, const char* name) {
+    const int n = int(s.fibers.size());
+    for (int i = 0; i < n; i++) {
+        Fiber& f = s.fibers[i];
+        if (f.stack.size() < 128 * 1024) f.stack.resize(128 * 1024);
+        getcontext(&f.uc);
+        f.uc.uc_stack.ss_sp = f.stack.data();
+        f.uc.uc_stack.ss_size = f.stack.size();
+        f.uc.uc_link = nullptr;
+        makecontext(&f.uc, fiber_entry, 0);
+    }
+    for (;;) {
+        bool allDone = true, progress = false;
+        for (int i = 0; i < n; i++) {
+            Fiber& f = s.fibers[i];
+            if (f.done) continue;
+            allDone = false;
+            if (f.waitKind == 1 && f.waitGen == s.blockGen) continue;
+            if (f.waitKind == 2 && f.waitGen == s.waves[f.c.tid.x / kWaveSize].gen) continue;
+            f.waitKind = 0;
+            s.cur = i;
+            ctx = f.c;
+            swapcontext(&s.main, &f.uc);
+            progress = true;
+        }
+        if (allDone) break;
+        if (!progress) {
+            std::fprintf(stderr, "emu: deadlock in kernel %s block %u:", name, s.fibers[0].c.bid.x);
+            for (int i = 0; i < n; i++)
+                std::fprintf(stderr, " t%d:%s", i, s.fibers[i].done ? "done" : s.fibers[i].waitKind == 1 ? "block" : "wave");
+            std::fprintf(stderr, "\n");
+            std::abort();
+        }
+    }
+}
+
+template <class K, class... Args>
+void launch_named(const char* name, K kernel, dim3 grid, dim3 block, size_t ldsBytes, Args... args) {
+    if (block.x > block_cap()) block.x = block_cap();
+    const unsigned nthreads = block.x;
+    lds().assign(ldsBytes + 16, 0);
+    auto call = [&]() { kernel(args...); };
+    Sched s;
+    s.body = [](void* p) { (*static_cast<decltype(call)*>(p))(); };
+    s.arg = &call;
+    sched() = &s;
+    s.fibers.resize(nthreads);
+    for (unsigned b = 0; b < grid.x; b++) {
+        for (auto& f : s.fibers) {
+            f.done = false;
+            f.waitKind = 0;
+            f.waitGen = 0;
+        }
+        s.blockGen = 0;
+        const unsigned nw = (nthreads + kWaveSize - 1) / kWaveSize;
+        s.waves.assign(nw, Wave{});
+        for (unsigned w = 0; w < nw; w++) s.waves[w].size = int(std::min<unsigned>(kWaveSize, nthreads - w * kWaveSize));
+        s.blockArrived = 0;
+        for (unsigned t = 0; t < nthreads; t++) {
+            s.fibers[t].c.tid = dim3(t);
+            s.fibers[t].c.bid = dim3(b);
+            s.fibers[t].c.bdim = block;
+            s.fibers[t].c.gdim = grid;
+        }
+        run_block(s, name);
+    }
+    sched() = nullptr;
+}
+
+}  // namespace emu
+
+#define threadIdx (emu::ctx.tid)
+#define blockIdx (emu::ctx.bid)
+#define blockDim (emu::ctx.bdim)
+#define gridDim (emu::ctx.gdim)
+
+#define hipLaunchKernelGGL(kernel, grid, block, lds, stream, ...) \
+    emu::launch_named(#kernel, kernel, grid, block, lds, __VA_ARGS__)
+
+inline void __syncthreads() { emu::block_barrier(); }
+#define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
+#define __builtin_amdgcn_wave_barrier() emu::wave_barrier()
+
+inline unsigned __lane_id() { return unsigned(emu::lane()); }
+inline unsigned long long __ballot(int pred) {
+    emu::Wave& w = emu::my_wave();
+    w.vals[emu::lane()] = pred ? 1ull : 0ull;
+    emu::wave_barrier();
+    unsigned long long m = 0;
+    for (int i = 0; i < w.size; i++)
+        if (w.vals[i]) m |= 1ull << i;
+    emu::wave_barrier();
+    return m;
+}
+template <class T>
+inline T __shfl(T v, int src, int width = 64) {
+    (void)width;
+    return emu::exchange(v, src & 63);
+}
+template <class T>
+inline T __shfl_xor(T v, int mask, int width = 64) {
+    (void)width;
+    return emu::exchange(v, (emu::lane() ^ mask) & 63);
+}
+template <class T>
+inline T __shfl_up(T v, unsigned delta, int width = 64) {
+    (void)width;
+    const int src = emu::lane() - int(delta);
+    const T r = emu::exchange(v, src < 0 ? emu::lane() : src);
+    return r;
+}
+
+inline int __popc(unsigned x) { return __builtin_popcount(x); }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz(unsigned(x)); }
+inline unsigned __brev(unsigned x) {
+    unsigned r = 0;
+    for (int i = 0; i < 32; i++) r |= ((x >> i) & 1u) << (31 - i);
+    return r;
+}
+using std::max;
+using std::min;
+
+template <class T>
+inline T atomicAdd(T* p, T v) {
+    return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
+}
+inline int atomicMin(int* p, int v) {
+    int cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+    return cur;
+}
+inline int atomicMax(int* p, int v) {
+    int cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (v > cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+    return cur;
+}
+inline int atomicExch(int* p, int v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicCAS(unsigned long long* p, unsigned long long expected, unsigned long long desired) {
+    __atomic_compare_exchange_n(p, &expected, desired, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+    return expected;
+}
+
+// runtime API
+struct emu_event {
+    double t;
+};
+inline hipError_t hipGetDeviceCount(int* c) {
+    *c = 1;
+    return hipSuccess;
+}
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+    *s = reinterpret_cast<hipStream_t>(1);
+    return hipSuccess;
+}
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t* e) {
+    *e = new emu_event{0};
+    return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t e) {
+    delete e;
+    return hipSuccess;
+}
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
+    e->t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    return hipSuccess;
+}
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
+    *ms = float(b->t - a->t);
+    return hipSuccess;
+}
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
+    *v = emu::num_cus();
+    return hipSuccess;
+}
+template <class T>
+inline hipError_t hipMalloc(T** p, size_t bytes) {
+    *p = static_cast<T*>(std::calloc(1, bytes ? bytes : 1));
+    return *p ? hipSuccess : hipErrorInvalidValue;
+}
+inline hipError_t hipFree(void* p) {
+    std::free(p);
+    return hipSuccess;
+}
+inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
+    *p = std::calloc(1, bytes ? bytes : 1);
+    return *p ? hipSuccess : hipErrorInvalidValue;
+}
+inline hipError_t hipHostFree(void* p) {
+    std::free(p);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
+    std::memcpy(d, s, n);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
+    std::memcpy(d, s, n);
+    return hipSuccess;
+}
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline const char* hipGetErrorString(hipError_t) { return "emulated HIP error"; }

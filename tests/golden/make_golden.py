@@ -1,0 +1,52 @@
+"""Writes tests/golden/golden.json: digests of the reference care-cpu output
+on seeded simulator inputs.
+
+Run in the container that holds the reference sources (care-cpu is built by
+oracle/ref.mk into oracle/_ref/).  Inputs are regenerated from their seeds by
+care_amd/_build/readsim; only the parameters and output digests are stored.
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from tests.cases import CASES  # noqa: E402
+
+CARE_CPU = os.path.join(ROOT, "oracle", "_ref", "care-cpu")
+READSIM = os.path.join(ROOT, "care_amd", "_build", "readsim")
+
+
+def md5(path):
+    h = hashlib.md5()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def main():
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, case in CASES.items():
+            reads = os.path.join(tmp, name + ".fastq")
+            truth = os.path.join(tmp, name + ".truth.fastq")
+            subprocess.run([READSIM, "gen", "--profile", case["profile"], "--seed", str(case["seed"]),
+                            "--genome", str(case["genome"]), "--out", reads, "--truth", truth], check=True)
+            odir = os.path.join(tmp, name + "_out")
+            subprocess.run([CARE_CPU, "-i", reads, "-o", "ref.fastq", "-d", odir, "-t", "8", "-m", "8G"]
+                           + case["args"], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            ev = subprocess.run([READSIM, "eval", "--orig", reads, "--corr", os.path.join(odir, "ref.fastq"),
+                                 "--truth", truth], check=True, stdout=subprocess.PIPE, text=True)
+            out[name] = {"input_md5": md5(reads), "output_md5": md5(os.path.join(odir, "ref.fastq")),
+                         "eval": json.loads(ev.stdout)}
+            print(name, out[name])
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,101 @@
+"""GPU parity: the HIP engine against the CPU oracle and the reference's
+golden outputs.
+
+Bit-exact bar everywhere: minhash signatures and candidate lists are
+integer work; corrected reads must be identical to care-cpu's (golden
+digests in tests/golden/golden.json, made by tests/golden/make_golden.py).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import care_amd
+from tests import helpers
+from tests.cases import CASES
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def se_plain(tmp_path_factory):
+    d = tmp_path_factory.mktemp("se_plain")
+    reads, truth = helpers.simulate(CASES["se_plain"], str(d))
+    rs = care_amd.ReadSet([reads], paired_end=False, qual_bits=0)
+    return rs
+
+
+@pytest.fixture(scope="module")
+def se_mixed(tmp_path_factory):
+    d = tmp_path_factory.mktemp("se_mixed")
+    reads, truth = helpers.simulate(CASES["se_mixed_q1"], str(d))
+    return care_amd.ReadSet([reads], paired_end=False, qual_bits=1)
+
+
+def _engine(rs, **kw):
+    opt = care_amd.default_options(**kw)
+    eng = care_amd.Engine(opt, device=0)
+    eng.upload_readset(rs)
+    return eng
+
+
+@pytest.mark.parametrize("which,k,h", [("se_plain", 20, 16), ("se_mixed", 18, 24)])
+def test_signatures_match_oracle(which, k, h, request):
+    rs = request.getfixturevalue(which)
+    _say("signatures %s k=%d h=%d n=%d" % (which, k, h, rs.n))
+    eng = _engine(rs, kmer_length=k, num_hash_maps=h, coverage=30.0)
+    n = min(rs.n, 3000)
+    got = eng.signatures(0, n)
+    orc = helpers.oracle()
+    seqs, lens = rs.seqs, rs.lengths
+    want = np.zeros((n, h), dtype=np.uint64)
+    for i in range(n):
+        orc.orc_signature(seqs[i].ctypes.data, int(lens[i]), k, h, want[i].ctypes.data)
+    eng.close()
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("which,k,h,cov,excl", [("se_plain", 20, 16, 30.0, 0), ("se_mixed", 18, 24, 50.0, 0)])
+def test_candidate_lists_match_oracle(which, k, h, cov, excl, request):
+    rs = request.getfixturevalue(which)
+    _say("query %s" % which)
+    eng = _engine(rs, kmer_length=k, num_hash_maps=h, coverage=cov, exclude_ambiguous=excl)
+    eng.build()
+    ids, offs = eng.query(0, rs.n)
+    orc = helpers.oracle()
+    seqs = np.ascontiguousarray(rs.seqs)
+    lens = np.ascontiguousarray(rs.lengths)
+    t = orc.orc_tables_build(seqs.ctypes.data, rs.pitch, lens.ctypes.data, rs.n, k, h, cov)
+    buf = np.zeros(h * 200, dtype=np.uint32)
+    mismatches = 0
+    for i in range(rs.n):
+        c = orc.orc_tables_query(t, seqs[i].ctypes.data, int(lens[i]), i, buf.ctypes.data, len(buf))
+        want = buf[:c]
+        got = ids[offs[i]:offs[i + 1]]
+        if not np.array_equal(got, want):
+            mismatches += 1
+    orc.orc_tables_free(t)
+    eng.close()
+    assert mismatches == 0
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_end_to_end_matches_reference(name, tmp_path):
+    case = CASES[name]
+    _say("end-to-end %s" % name)
+    reads, truth = helpers.simulate(case, str(tmp_path))
+    g = helpers.golden()[name]
+    assert helpers.md5(reads) == g["input_md5"], "simulator output changed"
+    out = tmp_path / "out"
+    proc = care_amd.run_care_gpu(["-i", reads, "-o", "gpu.fastq", "-d", str(out), "-g", "0", "-t", "8"] + case["args"],
+                                 stdout=__import__("subprocess").PIPE, stderr=__import__("subprocess").STDOUT, text=True,
+                                 timeout=300)
+    _say(proc.stdout[-2000:])
+    assert proc.returncode == 0
+    assert helpers.md5(str(out / "gpu.fastq")) == g["output_md5"]
