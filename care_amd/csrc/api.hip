@@ -156,6 +156,13 @@ struct care_context {
     PinnedBuf<unsigned long long> hU64;
 
     care_batch_stats stats{};
+
+    struct ForestStore {
+        DevBuf<ForestNode> nodes;
+        DevBuf<u32> treeStart;
+        DevForest dev;
+        bool loaded = false;
+    } forest[2];
 };
 
 namespace {
@@ -241,8 +248,9 @@ int care_context_create(int device, const care_options* opt, care_context** out)
         return fail(CARE_ERR_ARGUMENT, "kmer_length %d outside 1..31", opt->kmer_length);
     if (opt->num_hash_maps < 1 || opt->num_hash_maps > 64)
         return fail(CARE_ERR_ARGUMENT, "num_hash_maps %d outside 1..64", opt->num_hash_maps);
-    if (opt->correction_type != 0 || opt->correction_type_cands != 0)
-        return fail(CARE_ERR_ARGUMENT, "only correction type 0 (classic) is available in this build");
+    if ((opt->correction_type != 0 && opt->correction_type != 1) ||
+        (opt->correction_type_cands != 0 && opt->correction_type_cands != 1))
+        return fail(CARE_ERR_ARGUMENT, "correction types: 0 classic, 1 forest");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CARE_ERR_DEVICE, "no HIP device available");
     if (device < 0 || device >= count) return fail(CARE_ERR_ARGUMENT, "device %d outside 0..%d", device, count - 1);
@@ -543,8 +551,15 @@ int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_resu
     P.errPercentHalf = float(P.errPercent) * 0.5f;
     P.correctionType = o.correction_type;
     P.correctionTypeCands = o.correction_type_cands;
-    P.thresholdAnchor = o.threshold_anchor;
-    P.thresholdCands = o.threshold_cands;
+    P.estimatedCoverage = o.coverage;
+    if (o.correction_type == 1) {
+        if (!ctx->forest[0].loaded) return fail(CARE_ERR_STATE, "correction_type 1 needs an anchor forest (care_forest_load)");
+        P.forestA = ctx->forest[0].dev;
+    }
+    if (o.correct_candidates && o.correction_type_cands == 1) {
+        if (!ctx->forest[1].loaded) return fail(CARE_ERR_STATE, "correction_type_cands 1 needs a candidate forest (care_forest_load)");
+        P.forestC = ctx->forest[1].dev;
+    }
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         const int B = default_batch(ctx);
@@ -636,11 +651,116 @@ int care_last_batch_stats(care_context* ctx, care_batch_stats* stats) {
 }
 
 int care_forest_load(care_context* ctx, int which, const char* path, uint32_t max_trees) {
-    (void)which;
-    (void)path;
-    (void)max_trees;
-    if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
-    return fail(CARE_ERR_STATE, "forest classifier not available in this build");
+    // file format: include/forest.hpp:38-55 and 126-150, ml/care.py:198-282
+    if (!ctx || !path) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (which != 0 && which != 1) return fail(CARE_ERR_ARGUMENT, "which must be 0 (anchor) or 1 (candidates)");
+    const char* expected = which == 0 ? "13 extract_anchor_v3" : "14 extract_cands_v3";
+    const int nfeat = which == 0 ? 13 : 14;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(CARE_ERR_ARGUMENT, "Loading classifier file failed! %s", path);
+    std::vector<unsigned char> buf;
+    {
+        unsigned char tmp[65536];
+        size_t r;
+        while ((r = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + r);
+        std::fclose(f);
+    }
+    size_t pos = 0;
+    bool bad = false;
+    auto take = [&](void* dst, size_t n) {
+        if (pos + n > buf.size()) {
+            bad = true;
+            std::memset(dst, 0, n);
+            return;
+        }
+        std::memcpy(dst, buf.data() + pos, n);
+        pos += n;
+    };
+    uint64_t dlen = 0;
+    take(&dlen, 8);
+    if (bad || dlen > 4096) return fail(CARE_ERR_ARGUMENT, "%s: truncated classifier header", path);
+    std::string desc(size_t(dlen), '\0');
+    take(&desc[0], size_t(dlen));
+    if (bad || desc != expected)
+        return fail(CARE_ERR_ARGUMENT, "Classifier and extractor descriptors do not match! Expected: %s Received: %s",
+                    expected, desc.c_str());
+    uint32_t ntrees = 0;
+    take(&ntrees, 4);
+    if (bad) return fail(CARE_ERR_ARGUMENT, "%s: truncated classifier", path);
+    ntrees = std::min(ntrees, max_trees);
+    std::vector<ForestNode> nodes;
+    std::vector<u32> starts;
+    // iterative preorder parse with an explicit stack of (node, side)
+    for (uint32_t t = 0; t < ntrees && !bad; t++) {
+        uint32_t reserve = 0;
+        take(&reserve, 4);
+        const u32 base = u32(nodes.size());
+        starts.push_back(base);
+        struct Pending {
+            u32 node;
+            int side;  // 0: left pending, 1: right pending
+        };
+        std::vector<Pending> stack;
+        auto readNode = [&]() -> u32 {
+            ForestNode nd{};
+            take(&nd.att, 1);
+            take(&nd.split, 8);
+            take(&nd.flag, 1);
+            nodes.push_back(nd);
+            return u32(nodes.size() - 1 - base);
+        };
+        u32 root = readNode();
+        stack.push_back({root, 0});
+        while (!stack.empty() && !bad) {
+            Pending& pd = stack.back();
+            ForestNode& nd = nodes[base + pd.node];
+            if (nd.att >= nfeat) {
+                bad = true;
+                break;
+            }
+            const bool leaf = pd.side == 0 ? (nd.flag & 2) : (nd.flag & 1);
+            if (leaf) {
+                float p;
+                take(&p, 4);
+                u32 bits;
+                std::memcpy(&bits, &p, 4);
+                if (pd.side == 0) nd.lhs = bits;
+                else nd.rhs = bits;
+                if (pd.side == 0) pd.side = 1;
+                else stack.pop_back();
+            } else {
+                const u32 child = u32(nodes.size() - base);
+                if (pd.side == 0) {
+                    nodes[base + pd.node].lhs = child;
+                    pd.side = 1;
+                } else {
+                    nodes[base + pd.node].rhs = child;
+                    stack.pop_back();
+                }
+                readNode();
+                stack.push_back({child, 0});
+            }
+        }
+    }
+    if (bad) return fail(CARE_ERR_ARGUMENT, "%s: malformed or truncated classifier", path);
+    auto& st = ctx->forest[which];
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        st.nodes.alloc(std::max<size_t>(1, nodes.size()));
+        st.treeStart.alloc(std::max<size_t>(1, starts.size()));
+        if (!nodes.empty())
+            HIP_TRY(hipMemcpy(st.nodes.p, nodes.data(), sizeof(ForestNode) * nodes.size(), hipMemcpyHostToDevice));
+        if (!starts.empty())
+            HIP_TRY(hipMemcpy(st.treeStart.p, starts.data(), sizeof(u32) * starts.size(), hipMemcpyHostToDevice));
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    st.dev.nodes = st.nodes.p;
+    st.dev.treeStart = st.treeStart.p;
+    st.dev.numTrees = int(starts.size());
+    st.dev.threshold = which == 0 ? ctx->opt.threshold_anchor : ctx->opt.threshold_cands;
+    st.loaded = true;
+    return CARE_OK;
 }
 
 }  // extern "C"

@@ -267,6 +267,91 @@ __device__ __forceinline__ int choose_orientation(const Aln& f, const Aln& r, in
     return 3;
 }
 
+// include/forest.hpp:57-68, 211-226: mean leaf probability over the trees,
+// accepting as soon as the running mean reaches the threshold
+__device__ bool forest_decide(const DevForest& F, const float* f) {
+    float sum = 0.f;
+    for (int t = 0; t < F.numTrees; t++) {
+        const ForestNode* tree = F.nodes + F.treeStart[t];
+        u32 i = 0;
+        float p;
+        for (;;) {
+            const ForestNode nd = tree[i];
+            if (double(f[nd.att]) <= nd.split) {
+                if (nd.flag & 2) {
+                    p = __uint_as_float(nd.lhs);
+                    break;
+                }
+                i = nd.lhs;
+            } else {
+                if (nd.flag & 1) {
+                    p = __uint_as_float(nd.rhs);
+                    break;
+                }
+                i = nd.rhs;
+            }
+        }
+        sum += p;
+        if (sum / float(F.numTrees) >= F.threshold) return true;
+    }
+    return false;
+}
+
+struct ColumnStats {
+    float avgSupport, minSupport;
+    int minCov, maxCov;
+};
+
+// MSA properties over columns [first, last) (src/msa.cpp:304-331); one
+// thread, support summed in column order
+__device__ ColumnStats column_stats(const Lds& L, int first, int last) {
+    ColumnStats cs;
+    float mn = L.support[first];
+    float sum = 0.0f;
+    int mnc = L.coverage[first], mxc = L.coverage[first];
+    for (int c = first; c < last; c++) {
+        const float s = L.support[c];
+        mn = s < mn ? s : mn;
+        sum += s;
+        const int cv = L.coverage[c];
+        mnc = cv < mnc ? cv : mnc;
+        mxc = cv > mxc ? cv : mxc;
+    }
+    cs.avgSupport = sum / float(last - first);
+    cs.minSupport = mn;
+    cs.minCov = mnc;
+    cs.maxCov = mxc;
+    return cs;
+}
+
+// extract_anchor_v3 / extract_cands_v3 (include/classification.hpp:748-877);
+// origBase is the anchor's (or candidate's) base at column pos
+__device__ void column_features(const Lds& L, int cols, int pos, u32 origBase, const ColumnStats& cs, float cov,
+                                float* f) {
+    const float wA = L.wgt[0 * cols + pos], wC = L.wgt[1 * cols + pos], wG = L.wgt[2 * cols + pos],
+                wT = L.wgt[3 * cols + pos];
+    const float wsum = wA + wC + wG + wT;
+    const u32 cons = L.cons[pos];
+    const float wCons = L.wgt[cons * cols + pos];
+    const float cCons = float(L.cnt[cons * cols + pos]);
+    const float wOrig = L.wgt[origBase * cols + pos];
+    const float cOrig = float(L.cnt[origBase * cols + pos]);
+    const float coverage = float(L.coverage[pos]);
+    f[0] = cOrig / coverage;
+    f[1] = wOrig / wsum;
+    f[2] = wOrig / cOrig;
+    f[3] = cCons / coverage;
+    f[4] = L.support[pos];
+    f[5] = wCons / cCons;
+    f[6] = wsum / coverage;
+    f[7] = coverage / cov;
+    f[8] = wsum / cov;
+    f[9] = cs.avgSupport;
+    f[10] = cs.minSupport;
+    f[11] = float(cs.maxCov) / cov;
+    f[12] = float(cs.minCov) / cov;
+}
+
 }  // namespace
 
 __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) {
@@ -568,22 +653,17 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             msa = build_msa(P, L, cur, m, la, &sMin, &sMax);
         }
 
-        // ---- phase D: anchor correction (src/msa.cpp:304-453) ----
-        if (m > 0) {
+        // ---- phase D: anchor correction (src/msa.cpp:304-453, include/corrector.hpp:1423-1472) ----
+        // the forest path corrects even when refinement removed every
+        // candidate (correctAnchorClf sets isCorrected unconditionally)
+        const bool forestA = P.correctionType == 1;
+        if (m > 0 || forestA) {
             __shared__ float sMinSupport, sAvgSupport;
-            __shared__ int sMinCov, sHQ, sByCons;
+            __shared__ int sMinCov, sMaxCov, sHQ, sByCons;
             if (tid == 0) {
-                float mn = L.support[msa.ab];
-                float sum = 0.0f;
-                int mc = L.coverage[msa.ab];
-                for (int i = 0; i < la; i++) {
-                    const float s = L.support[msa.ab + i];
-                    mn = s < mn ? s : mn;
-                    sum += s;
-                    const int cv = L.coverage[msa.ab + i];
-                    mc = cv < mc ? cv : mc;
-                }
-                const float avg = sum / float(la);
+                const ColumnStats cs = column_stats(L, msa.ab, msa.ab + la);
+                const float mn = cs.minSupport, avg = cs.avgSupport;
+                const int mc = cs.minCov;
                 const bool allGood = fgeq(avg, P.avgSupportThreshold) && fgeq(mn, P.minSupportThreshold) &&
                                      fgeq(float(mc), P.minCoverageThreshold);
                 int hq = 0;
@@ -598,39 +678,85 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 sMinSupport = mn;
                 sAvgSupport = avg;
                 sMinCov = mc;
+                sMaxCov = cs.maxCov;
                 sHQ = hq;
                 sByCons = allGood;
             }
             __syncthreads();
             const bool byCons = sByCons != 0;
+            const bool hq = sHQ != 0;
+            ColumnStats acs;
+            acs.avgSupport = sAvgSupport;
+            acs.minSupport = sMinSupport;
+            acs.minCov = sMinCov;
+            acs.maxCov = sMaxCov;
             for (int i = tid; i < la; i += blockDim.x) {
                 const int col = msa.ab + i;
+                const u32 orig = base_at(L.aw, i);
                 u32 b;
-                if (byCons) b = L.cons[col];
-                else b = (L.support[col] > 0.90f && L.origCov[col] <= 2) ? u32(L.cons[col]) : base_at(L.aw, i);
+                if (forestA) {
+                    b = L.cons[col];
+                    if (!hq && b != orig) {
+                        float f[14];
+                        column_features(L, msa.cols, col, orig, acs, P.estimatedCoverage, f);
+                        if (!forest_decide(P.forestA, f)) b = orig;
+                    }
+                } else if (byCons) {
+                    b = L.cons[col];
+                } else {
+                    b = (L.support[col] > 0.90f && L.origCov[col] <= 2) ? u32(L.cons[col]) : orig;
+                }
                 outSeq[i] = "ACGT"[b];
             }
-            status = u8(1 | (sHQ ? 2 : 0));
+            status = u8(1 | (hq ? 2 : 0));
 
-            // ---- phase E: candidate correction (src/msa.cpp:457-522) ----
-            if (sHQ && P.correctCandidates) {
+            // ---- phase E: candidate correction (src/msa.cpp:457-522,
+            // include/corrector.hpp:1539-1623) ----
+            if (hq && P.correctCandidates) {
                 const int ae = msa.ab + la;
+                const bool forestC = P.correctionTypeCands == 1;
                 for (int c = wave; c < m; c += nwaves) {
                     const CandMeta cm = cur[c];
                     const int qb = msa.ab + cm.shift;
                     const int qe = qb + cm.len;
-                    if (!(msa.ab - P.newColumns <= qb && qb <= msa.ab + P.newColumns && qe <= ae + P.newColumns)) continue;
+                    const bool inRange = forestC ? (qb >= msa.ab - P.newColumns && qe <= ae + P.newColumns)
+                                                 : (msa.ab - P.newColumns <= qb && qb <= msa.ab + P.newColumns &&
+                                                    qe <= ae + P.newColumns);
+                    if (!inRange) continue;
                     int slot = 0;
                     if (lane == 0) slot = atomicAdd(P.candOutCount, 1);
                     slot = __shfl(slot, 0, kWave);
                     if (slot >= P.candOutCap) continue;
                     if (lane == 0) P.candOutIds[slot] = cm.id;
+                    ColumnStats ccs{};
+                    if (forestC) {
+                        if (lane == 0) ccs = column_stats(L, qb, qe);
+                        ccs.avgSupport = __shfl(ccs.avgSupport, 0, kWave);
+                        ccs.minSupport = __shfl(ccs.minSupport, 0, kWave);
+                        ccs.minCov = __shfl(ccs.minCov, 0, kWave);
+                        ccs.maxCov = __shfl(ccs.maxCov, 0, kWave);
+                    }
+                    // jaccard of the anchor and candidate column ranges
+                    const float jac = float(min(ae, qe) - max(msa.ab, qb)) / float(max(ae, qe) - min(msa.ab, qb));
                     char* dst = P.candOutSeq + u64(slot) * P.outPitch;
                     const bool rc = cm.flag == 2;
+                    const u32* enc = rs.seqs + u64(cm.id) * rs.pitch;
                     for (int i = lane; i < cm.len; i += kWave) {
-                        // forward strand of the corrected candidate
-                        const u32 b = rc ? 3u - L.cons[qb + cm.len - 1 - i] : L.cons[qb + i];
-                        dst[i] = "ACGT"[b];
+                        // position i of the candidate in MSA orientation, written
+                        // to its forward-strand position
+                        const int col = qb + i;
+                        u32 b = L.cons[col];
+                        if (forestC) {
+                            const u32 orig = oriented_base(enc, cm.len, i, rc);
+                            if (b != orig) {
+                                float f[14];
+                                column_features(L, msa.cols, col, orig, ccs, P.estimatedCoverage, f);
+                                f[13] = jac;
+                                if (!forest_decide(P.forestC, f)) b = orig;
+                            }
+                        }
+                        const int out = rc ? cm.len - 1 - i : i;
+                        dst[out] = "ACGT"[rc ? 3u - b : b];
                     }
                 }
             }

@@ -63,6 +63,27 @@ struct CandMeta {
 };
 static_assert(sizeof(CandMeta) == 16, "CandMeta layout");
 
+// Random-forest node (include/forest.hpp:22-37): a split on feature `att`
+// at `split`; flag bit 1 / bit 0 mark the left / right child as a leaf, in
+// which case lhs / rhs hold the leaf probability's float bits, otherwise
+// the child's index within the tree.
+struct ForestNode {
+    double split;
+    u32 lhs;
+    u32 rhs;
+    u8 att;
+    u8 flag;
+    u8 pad[6];
+};
+static_assert(sizeof(ForestNode) == 24, "ForestNode layout");
+
+struct DevForest {
+    const ForestNode* nodes = nullptr;
+    const u32* treeStart = nullptr;
+    int numTrees = 0;
+    float threshold = 0.5f;
+};
+
 // Scalar parameters of the correction kernel, precomputed on the host in
 // the reference's float arithmetic.
 struct CorrectParams {
@@ -104,10 +125,9 @@ struct CorrectParams {
     // forest / learned gate
     int correctionType;
     int correctionTypeCands;
-    const void* forestAnchor;
-    const void* forestCands;
-    float thresholdAnchor;
-    float thresholdCands;
+    DevForest forestA;
+    DevForest forestC;
+    float estimatedCoverage;
 };
 
 // launchers (all asynchronous on `stream`)

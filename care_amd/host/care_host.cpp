@@ -309,6 +309,7 @@ struct HostOptions {
     std::vector<int> gpus;
     int batchsize = 0;
     std::string forestAnchor, forestCands;
+    uint32_t maxTreesAnchor = 0xFFFFFFFFu, maxTreesCands = 0xFFFFFFFFu;
     bool haveCoverage = false, havePairmode = false;
 };
 
@@ -379,6 +380,13 @@ HostOptions parseOptions(int argc, char** argv) {
         else if (a == "--thresholdCands") o.eng.threshold_cands = std::stof(value());
         else if (a == "--pairedFilterThreshold") o.eng.paired_filter_threshold = std::stof(value());
         else if (a == "--fixedNumberOfReads") o.fixedNumberOfReads = std::stoull(value());
+        else if (a == "--maxForestTreesAnchor") {
+            const int v = std::stoi(value());
+            if (v > 0) o.maxTreesAnchor = uint32_t(v);  // src/options.cpp:136-145
+        } else if (a == "--maxForestTreesCands") {
+            const int v = std::stoi(value());
+            if (v > 0) o.maxTreesCands = uint32_t(v);
+        }
         else if (a == "--batchsize") o.batchsize = std::stoi(value());
         else if (a == "--gzoutput") o.gzoutput = flag();
         else if (a == "-p" || a == "--showProgress") o.showProgress = flag();
@@ -386,7 +394,7 @@ HostOptions parseOptions(int argc, char** argv) {
         else if (a == "-m" || a == "--memTotal" || a == "--memHashtables" || a == "--tempdir" || a == "--hashloadfactor" ||
                  a == "--warpcore" || a == "--replicateGpuData" || a == "--save-preprocessedreads-to" ||
                  a == "--load-preprocessedreads-from" || a == "--save-hashtables-to" || a == "--load-hashtables-from" ||
-                 a == "--maxForestTreesAnchor" || a == "--maxForestTreesCands" || a == "--singlehash" ||
+                 a == "--singlehash" ||
                  a == "--samplingRateAnchor" || a == "--samplingRateCands")
             (void)value();
         else throw std::runtime_error("unknown option " + a);
@@ -555,6 +563,9 @@ int runMain(int argc, char** argv) {
     check(care_context_create(o.gpus[0], &o.eng, &ctx));
     std::unique_ptr<care_context, int (*)(care_context*)> guard(ctx, care_context_destroy);
     if (o.batchsize > 0) check(care_set_launch_config(ctx, (o.batchsize + 1) & ~1, 0));
+    if (o.eng.correction_type == 1) check(care_forest_load(ctx, 0, o.forestAnchor.c_str(), o.maxTreesAnchor));
+    if (o.eng.correct_candidates && o.eng.correction_type_cands == 1)
+        check(care_forest_load(ctx, 1, o.forestCands.c_str(), o.maxTreesCands));
     check(care_readstore_upload(ctx, rs->n, rs->maxLen, rs->seqs.data(), rs->pitch, rs->lens.data(), rs->ambig.data(),
                                 o.eng.use_quality ? rs->quals.data() : nullptr, rs->qpitch, rs->qbits));
     check(care_minhash_build(ctx));

@@ -624,62 +624,88 @@ int chooseOrientation(const Alignment& f, const Alignment& r, int la, float minO
 }
 
 // ---------------------------------------------------------------------------
-// Forest classifier (include/forest.hpp:22-227, include/classification.hpp:748-892)
+// Forest classifier (include/forest.hpp:22-227): trees stored in preorder,
+// node = att u8, split f64, flag u8 (bit1: left child is a leaf, bit0: right
+// child is a leaf), leaves hold P(accept) as f32.  decide() averages leaf
+// probabilities over trees and accepts as soon as the running mean reaches
+// the threshold.
 // ---------------------------------------------------------------------------
 struct Forest {
     struct Node {
         std::uint8_t att = 0;
-        std::uint8_t flag = 0;  // bit0: lhs is leaf, bit1: rhs is leaf
+        std::uint8_t flag = 0;
         double split = 0;
         union {
             std::uint32_t idx;
             float prob;
-        } lhs, rhs;
+        } lhs{}, rhs{};
     };
     std::vector<std::vector<Node>> trees;
+    float thresh = 0.5f;
 
     template <class T>
     static T readT(std::istream& is) {
         T v;
-        is.read(reinterpret_cast<char*>(&v), sizeof(T));
+        char buf[sizeof(T)];
+        is.read(buf, sizeof(T));
         if (!is) throw std::runtime_error("truncated forest file");
+        std::memcpy(&v, buf, sizeof(T));
         return v;
     }
-    void populate(std::istream& is, std::vector<Node>& tree) {
-        Node& node = tree.emplace_back();
-        node.att = readT<std::uint8_t>(is);
-        node.split = readT<double>(is);
-        const std::uint8_t lhsLeaf = readT<std::uint8_t>(is);
-        if (lhsLeaf) {
-            node.flag |= 1;
-            node.lhs.prob = readT<float>(is);
+    static void populate(std::istream& is, std::vector<Node>& tree) {
+        const std::size_t me = tree.size();
+        tree.emplace_back();
+        tree[me].att = readT<std::uint8_t>(is);
+        tree[me].split = readT<double>(is);
+        tree[me].flag = readT<std::uint8_t>(is);
+        if (tree[me].flag & 2) {
+            tree[me].lhs.prob = readT<float>(is);
         } else {
-            const std::size_t me = tree.size() - 1;
-            const std::uint32_t child = std::uint32_t(tree.size());
-            tree[me].lhs.idx = child;
+            tree[me].lhs.idx = std::uint32_t(tree.size());
             populate(is, tree);
         }
-        const std::size_t me2 = &node - tree.data() < 0 ? 0 : 0;
-        (void)me2;
+        if (tree[me].flag & 1) {
+            tree[me].rhs.prob = readT<float>(is);
+        } else {
+            tree[me].rhs.idx = std::uint32_t(tree.size());
+            populate(is, tree);
+        }
     }
-    float leafProb(const std::vector<Node>& tree, const float* f) const {
+    void load(const std::string& path, const std::string& expectedDesc, std::uint32_t maxTrees, float t) {
+        std::ifstream is(path, std::ios::binary);
+        if (!is) throw std::runtime_error("Loading classifier file failed! " + path);
+        const auto len = readT<std::uint64_t>(is);
+        std::string desc(len, '\0');
+        is.read(&desc[0], std::streamsize(len));
+        if (desc != expectedDesc)
+            throw std::runtime_error("Classifier and extractor descriptors do not match! Expected: " + expectedDesc +
+                                     " Received: " + desc);
+        const auto n = readT<std::uint32_t>(is);
+        trees.assign(std::min(n, maxTrees), {});
+        for (auto& tree : trees) {
+            tree.reserve(readT<std::uint32_t>(is));
+            populate(is, tree);
+        }
+        thresh = t;
+    }
+    static float leaf(const std::vector<Node>& tree, const float* f) {
         std::size_t i = 0;
         for (;;) {
             const Node& n = tree[i];
-            if (f[n.att] <= n.split) {
-                if (n.flag & 1) return n.lhs.prob;
+            if (double(f[n.att]) <= n.split) {
+                if (n.flag & 2) return n.lhs.prob;
                 i = n.lhs.idx;
             } else {
-                if (n.flag & 2) return n.rhs.prob;
+                if (n.flag & 1) return n.rhs.prob;
                 i = n.rhs.idx;
             }
         }
     }
-    bool decide(const float* f, float t) const {
-        float prob = 0.f;
+    bool decide(const float* f) const {
+        float sum = 0.f;
         for (const auto& tree : trees) {
-            prob += leafProb(tree, f);
-            if (prob / trees.size() >= t) return true;
+            sum += leaf(tree, f);
+            if (sum / float(trees.size()) >= thresh) return true;
         }
         return false;
     }
@@ -769,20 +795,21 @@ struct MSA {
     };
 
     // src/msa.cpp:304-376
-    Props properties(float err, float cov, float mcov) const {
+    Props properties(float err, float cov, float mcov) const { return properties(anchorBegin, anchorEnd, err, cov, mcov); }
+    Props properties(int first, int last, float err, float cov, float mcov) const {
         Props p;
         const float avgT = 1.0f - 1.0f * err;
         // care-cpu (built -O3 -march=native) evaluates every 1 - 3x threshold
         // as one fused multiply-add
         const float minT = std::fma(-3.0f, err, 1.0f);
         const float covT = mcov / 6.0f * cov;
-        const int dist = anchorEnd - anchorBegin;
-        p.min_support = *std::min_element(support.begin() + anchorBegin, support.begin() + anchorEnd);
+        const int dist = last - first;
+        p.min_support = *std::min_element(support.begin() + first, support.begin() + last);
         float sum = 0.0f;
-        for (int c = anchorBegin; c < anchorEnd; c++) sum += support[c];
+        for (int c = first; c < last; c++) sum += support[c];
         p.avg_support = sum / dist;
-        p.min_coverage = *std::min_element(coverage.begin() + anchorBegin, coverage.begin() + anchorEnd);
-        p.max_coverage = *std::max_element(coverage.begin() + anchorBegin, coverage.begin() + anchorEnd);
+        p.min_coverage = *std::min_element(coverage.begin() + first, coverage.begin() + last);
+        p.max_coverage = *std::max_element(coverage.begin() + first, coverage.begin() + last);
         const bool allGood = fgeq(p.avg_support, avgT) && fgeq(p.min_support, minT) && fgeq(float(p.min_coverage), covT);
         p.isHQ = false;
         if (allGood) {
@@ -1077,21 +1104,22 @@ struct Corrector {
         const MSA::Props props = msa.properties(opt.estimatedErrorrate, opt.estimatedCoverage, opt.m_coverage);
         std::string corrected;
         bool isCorrected = false;
-        if (msa.nCandidates > 0) {
-            if (opt.correctionType == 1 && forestA) {
-                corrected.assign(msa.consensus.begin() + msa.anchorBegin, msa.consensus.begin() + msa.anchorBegin + la);
-                if (!props.isHQ) {
-                    for (int i = 0; i < la; i++) {
-                        if (anchor[i] != msa.consensus[msa.anchorBegin + i]) {
-                            float f[13];
-                            anchorFeatures(f, msa, props, anchor, i);
-                            if (!forestA->decide(f, opt.thresholdAnchor)) corrected[i] = anchor[i];
-                        }
+        if (opt.correctionType == 1) {
+            // correctAnchorClf: consensus, non-HQ columns that differ from
+            // the anchor revert unless the classifier accepts
+            corrected.assign(msa.consensus.begin() + msa.anchorBegin, msa.consensus.begin() + msa.anchorBegin + la);
+            if (!props.isHQ) {
+                for (int i = 0; i < la; i++) {
+                    if (anchor[i] != msa.consensus[msa.anchorBegin + i]) {
+                        float f[13];
+                        anchorFeatures(f, msa, props, i);
+                        if (!forestA->decide(f)) corrected[i] = anchor[i];
                     }
                 }
-            } else {
-                corrected = msa.correctedAnchor(props, opt.estimatedErrorrate, opt.estimatedCoverage, opt.m_coverage);
             }
+            isCorrected = true;
+        } else if (msa.nCandidates > 0) {
+            corrected = msa.correctedAnchor(props, opt.estimatedErrorrate, opt.estimatedCoverage, opt.m_coverage);
             isCorrected = true;
         }
         out.hq = props.isHQ && isCorrected;
@@ -1111,21 +1139,29 @@ struct Corrector {
             makeEdits(out.anchor, corrected, anchor.data(), rs.ambig[id] != 0);
         }
 
-        // candidate correction (src/msa.cpp:457-522, include/corrector.hpp:1504-1623)
-        if (props.isHQ && opt.correctCandidates && isCorrected) {
+        // candidate correction: classic (src/msa.cpp:457-522) takes candidates
+        // starting within +-ncc columns of the anchor and ending at most ncc
+        // columns after it; forest (include/corrector.hpp:1539-1623) drops
+        // the bound on the start after the anchor start
+        if (props.isHQ && opt.correctCandidates) {
             for (int c = 0; c < msa.nCandidates; c++) {
                 const int qb = msa.anchorBegin + cands[c].al.shift;
                 const int len = cands[c].len;
                 const int qe = qb + len;
                 const int ncc = opt.new_columns_to_correct;
-                if (!(msa.anchorBegin - ncc <= qb && qb <= msa.anchorBegin + ncc && qe <= msa.anchorEnd + ncc)) continue;
+                const bool forest = opt.correctionTypeCands == 1;
+                const bool inRange = forest ? (qb >= msa.anchorBegin - ncc && qe <= msa.anchorEnd + ncc)
+                                            : (msa.anchorBegin - ncc <= qb && qb <= msa.anchorBegin + ncc &&
+                                               qe <= msa.anchorEnd + ncc);
+                if (!inRange) continue;
                 std::string seq(msa.consensus.begin() + qb, msa.consensus.begin() + qe);
-                if (opt.correctionTypeCands == 1 && forestC) {
+                if (forest) {
+                    const MSA::Props cprops = msa.properties(qb, qe, opt.estimatedErrorrate, opt.estimatedCoverage, opt.m_coverage);
                     for (int i = 0; i < len; i++) {
                         if (cands[c].chars[i] != msa.consensus[qb + i]) {
                             float f[14];
-                            candFeatures(f, msa, props, cands[c].chars, c, qb, i);
-                            if (!forestC->decide(f, opt.thresholdCands)) seq[i] = cands[c].chars[i];
+                            candFeatures(f, msa, cprops, cands[c].chars[i], qb, qe, i);
+                            if (!forestC->decide(f)) seq[i] = cands[c].chars[i];
                         }
                     }
                 }
@@ -1147,50 +1183,74 @@ struct Corrector {
         return out;
     }
 
-    // extract_anchor_v3 (include/classification.hpp:748-810)
-    void anchorFeatures(float* f, const MSA& msa, const MSA::Props& p, const std::string& anchor, int i) const {
-        const int c = msa.anchorBegin + i;
-        const char orig = anchor[i];
-        const char cons = msa.consensus[c];
-        auto w = [&](char b) {
-            switch (b) {
-                case 'A': return msa.wA[c];
-                case 'C': return msa.wC[c];
-                case 'G': return msa.wG[c];
-                default: return msa.wT[c];
-            }
-        };
-        auto cnt = [&](char b) {
-            switch (b) {
-                case 'A': return msa.cA[c];
-                case 'C': return msa.cC[c];
-                case 'G': return msa.cG[c];
-                default: return msa.cT[c];
-            }
-        };
-        const float cov = float(msa.coverage[c]);
-        f[0] = float(orig == 'A');
-        f[1] = float(orig == 'C');
-        f[2] = float(orig == 'G');
-        f[3] = float(orig == 'T');
-        f[4] = float(cons == 'A');
-        f[5] = float(cons == 'C');
-        f[6] = float(cons == 'G');
-        f[7] = float(cons == 'T');
-        f[8] = w(orig);
-        f[9] = w(cons);
-        f[10] = float(cnt(orig)) / cov;
-        f[11] = float(cnt(cons)) / cov;
-        f[12] = p.avg_support;
-        (void)p;
+    float weightOf(const MSA& msa, char b, int c) const {
+        switch (b) {
+            case 'A': return msa.wA[c];
+            case 'C': return msa.wC[c];
+            case 'G': return msa.wG[c];
+            case 'T': return msa.wT[c];
+            default: return 0.f;
+        }
     }
-    void candFeatures(float* f, const MSA& msa, const MSA::Props& p, const std::string& chars, int cand, int qb, int i) const {
-        (void)cand;
-        const int c = qb + i;
-        const char orig = chars[i];
-        std::string a(1, orig);
-        anchorFeatures(f, msa, p, a, c - msa.anchorBegin);
-        f[13] = 0.f;
+    float countOf(const MSA& msa, char b, int c) const {
+        switch (b) {
+            case 'A': return float(msa.cA[c]);
+            case 'C': return float(msa.cC[c]);
+            case 'G': return float(msa.cG[c]);
+            case 'T': return float(msa.cT[c]);
+            default: return 0.f;
+        }
+    }
+
+    // extract_anchor_v3 (include/classification.hpp:748-800)
+    void anchorFeatures(float* f, const MSA& msa, const MSA::Props& p, int i) const {
+        const int pos = msa.anchorBegin + i;
+        const float cov = opt.estimatedCoverage;
+        const float wsum = msa.wA[pos] + msa.wC[pos] + msa.wG[pos] + msa.wT[pos];
+        const float wCons = weightOf(msa, msa.consensus[pos], pos);
+        const float cCons = countOf(msa, msa.consensus[pos], pos);
+        const int coverage = msa.coverage[pos];
+        f[0] = float(msa.origCoverages[pos]) / float(coverage);
+        f[1] = msa.origWeights[pos] / wsum;
+        f[2] = msa.origWeights[pos] / float(msa.origCoverages[pos]);
+        f[3] = cCons / float(coverage);
+        f[4] = msa.support[pos];
+        f[5] = wCons / cCons;
+        f[6] = wsum / float(coverage);
+        f[7] = float(coverage) / cov;
+        f[8] = wsum / cov;
+        f[9] = p.avg_support;
+        f[10] = p.min_support;
+        f[11] = float(p.max_coverage) / cov;
+        f[12] = float(p.min_coverage) / cov;
+    }
+
+    // extract_cands_v3 (include/classification.hpp:803-877); p are the MSA
+    // properties over the candidate's columns [qb, qe)
+    void candFeatures(float* f, const MSA& msa, const MSA::Props& p, char origBase, int qb, int qe, int i) const {
+        const int pos = qb + i;
+        const float cov = opt.estimatedCoverage;
+        const float wsum = msa.wA[pos] + msa.wC[pos] + msa.wG[pos] + msa.wT[pos];
+        const float wCons = weightOf(msa, msa.consensus[pos], pos);
+        const float cCons = countOf(msa, msa.consensus[pos], pos);
+        const float wOrig = weightOf(msa, origBase, pos);
+        const float cOrig = countOf(msa, origBase, pos);
+        const int coverage = msa.coverage[pos];
+        f[0] = cOrig / float(coverage);
+        f[1] = wOrig / wsum;
+        f[2] = wOrig / cOrig;
+        f[3] = cCons / float(coverage);
+        f[4] = msa.support[pos];
+        f[5] = wCons / cCons;
+        f[6] = wsum / float(coverage);
+        f[7] = float(coverage) / cov;
+        f[8] = wsum / cov;
+        f[9] = p.avg_support;
+        f[10] = p.min_support;
+        f[11] = float(p.max_coverage) / cov;
+        f[12] = float(p.min_coverage) / cov;
+        const int ab = msa.anchorBegin, ae = msa.anchorEnd;
+        f[13] = float(std::min(ae, qe) - std::max(ab, qb)) / float(std::max(ae, qe) - std::min(ab, qb));
     }
 
     // Corrects anchors [0, numAnchors); SE one at a time, PE in mate pairs.
@@ -1347,6 +1407,13 @@ Options parseArgs(int argc, char** argv) {
         else if (a == "--pairedFilterThreshold") o.pairedFilterThreshold = std::stof(next());
         else if (a == "--fixedNumberOfReads") o.fixedNumberOfReads = std::stoull(next());
         else if (a == "--dump") o.dumpFile = next();
+        else if (a == "--maxForestTreesAnchor") {
+            const int v = std::stoi(next());
+            o.maxTreesAnchor = v > 0 ? std::uint32_t(v) : 0xFFFFFFFFu;
+        } else if (a == "--maxForestTreesCands") {
+            const int v = std::stoi(next());
+            o.maxTreesCands = v > 0 ? std::uint32_t(v) : 0xFFFFFFFFu;
+        }
         else if (a == "-m" || a == "--memTotal" || a == "--memHashtables" || a == "--tempdir" || a == "--hashloadfactor" ||
                  a == "-g" || a == "--gpu" || a == "--batchsize" || a == "--warpcore")
             (void)next();
@@ -1388,6 +1455,16 @@ int runMain(int argc, char** argv) {
     const double t1 = now();
 
     Corrector corr(opt, rs, tables);
+    // include/classification.hpp:72-74, descriptors of the v3 extractors
+    Forest fa, fc;
+    if (opt.correctionType == 1) {
+        fa.load(opt.forestAnchor, "13 extract_anchor_v3", opt.maxTreesAnchor, opt.thresholdAnchor);
+        corr.forestA = &fa;
+    }
+    if (opt.correctionTypeCands == 1) {
+        fc.load(opt.forestCands, "14 extract_cands_v3", opt.maxTreesCands, opt.thresholdCands);
+        corr.forestC = &fc;
+    }
     std::size_t numAnchors = opt.fixedNumberOfReads ? std::min(opt.fixedNumberOfReads, rs.n) : rs.n;
     std::vector<Corrector::Output> outs;
     std::vector<AnchorDump> dumps;

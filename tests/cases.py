@@ -21,8 +21,34 @@ CASES = {
     "pe_candidates": {"profile": "large-pe", "seed": 104, "genome": 120000,
                       "args": ["-c", "40", "--pairmode", "PE", "-q", "--candidateCorrection", "-k", "20", "-h", "48",
                                "--correctionQualityLabels"]},
+    # forest classifier for anchors and candidates (random forests written by
+    # tests/forest_files.py from the given seeds)
+    "pe_forest": {"profile": "repeat-pe", "seed": 106, "genome": 60000,
+                  "args": ["-c", "30", "--pairmode", "PE", "-q", "-k", "20", "-h", "32", "--candidateCorrection",
+                           "--correctionType", "1", "--correctionTypeCands", "1"],
+                  "forest": {"anchor": 1, "cands": 2}},
+    "se_forest_anchor": {"profile": "uniform-se", "seed": 107, "genome": 40000,
+                         "args": ["-c", "30", "--pairmode", "SE", "-q", "-k", "20", "-h", "16", "--correctionType", "1"],
+                         "forest": {"anchor": 3}},
     # variable length without excluding ambiguous reads, 1-bit qualities
     "se_mixed_q1": {"profile": "mixed-se", "seed": 105, "genome": 80000,
                     "args": ["-c", "50", "--pairmode", "SE", "-q", "--qualityScoreBits", "1", "-k", "18", "-h", "24",
                              "--candidateCorrection"]},
 }
+
+
+def case_args(case, directory):
+    """Command-line options of a case; writes its forest files into directory."""
+    import os
+    from tests.forest_files import ANCHOR_DESC, CANDS_DESC, write_forest
+    args = list(case["args"])
+    forest = case.get("forest", {})
+    if "anchor" in forest:
+        path = os.path.join(directory, "anchor.rf")
+        write_forest(path, ANCHOR_DESC, forest["anchor"])
+        args += ["--ml-forestfile", path]
+    if "cands" in forest:
+        path = os.path.join(directory, "cands.rf")
+        write_forest(path, CANDS_DESC, forest["cands"])
+        args += ["--ml-cands-forestfile", path]
+    return args

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
+#include <csetjmp>
 #include <ucontext.h>
 #include <chrono>
 #include <cmath>
@@ -64,6 +65,8 @@ inline ThreadCtx ctx;
 
 struct Fiber {
     ucontext_t uc;
+    jmp_buf jb;
+    bool started = false;
     std::vector<char> stack;
     ThreadCtx c;
     bool done = false;
@@ -82,6 +85,7 @@ struct Sched {
     std::vector<Fiber> fibers;
     std::vector<Wave> waves;
     ucontext_t main;
+    jmp_buf mainJb;
     int cur = -1;
     int blockArrived = 0;
     unsigned long blockGen = 0;
@@ -102,10 +106,12 @@ inline int num_cus() { return 2; }
 inline int lane() { return int(ctx.tid.x % kWaveSize); }
 inline Wave& my_wave() { return sched()->waves[ctx.tid.x / kWaveSize]; }
 
+// Fibers enter through makecontext/swapcontext once; later switches use
+// _setjmp/_longjmp, which do not save the signal mask (no system call).
 inline void yield_to_scheduler() {
     Sched* s = sched();
     Fiber& f = s->fibers[s->cur];
-    swapcontext(&f.uc, &s->main);
+    if (_setjmp(f.jb) == 0) _longjmp(s->mainJb, 1);
 }
 
 inline void block_barrier() {
@@ -161,10 +167,9 @@ inline T exchange(T v, int src) {
 
 inline void fiber_entry() {
     Sched* s = sched();
-    Fiber& f = s->fibers[s->cur];
     s->body(s->arg);
-    f.done = true;
-    swapcontext(&f.uc, &s->main);
+    s->fibers[s->cur].done = true;
+    _longjmp(s->mainJb, 1);
 }
 
 // Kernels of the engine accept any block size that is a multiple of 64;
@@ -180,12 +185,13 @@ inline void run_block(Sched& s, const char* name) {
     const int n = int(s.fibers.size());
     for (int i = 0; i < n; i++) {
         Fiber& f = s.fibers[i];
-        if (f.stack.size() < 128 * 1024) f.stack.resize(128 * 1024);
+        if (f.stack.size() < 128 * 1024) f.stack.resize(128 * 1024);  // kept across launches
         getcontext(&f.uc);
         f.uc.uc_stack.ss_sp = f.stack.data();
         f.uc.uc_stack.ss_size = f.stack.size();
         f.uc.uc_link = nullptr;
         makecontext(&f.uc, fiber_entry, 0);
+        f.started = false;
     }
     for (;;) {
         bool allDone = true, progress = false;
@@ -198,7 +204,14 @@ inline void run_block(Sched& s, const char* name) {
             f.waitKind = 0;
             s.cur = i;
             ctx = f.c;
-            swapcontext(&s.main, &f.uc);
+            if (_setjmp(s.mainJb) == 0) {
+                if (!f.started) {
+                    f.started = true;
+                    swapcontext(&s.main, &f.uc);
+                } else {
+                    _longjmp(f.jb, 1);
+                }
+            }
             progress = true;
         }
         if (allDone) break;
@@ -212,13 +225,18 @@ inline void run_block(Sched& s, const char* name) {
     }
 }
 
+inline Sched& sched_storage() {
+    static Sched s;
+    return s;
+}
+
 template <class K, class... Args>
 void launch_named(const char* name, K kernel, dim3 grid, dim3 block, size_t ldsBytes, Args... args) {
     if (block.x > block_cap()) block.x = block_cap();
     const unsigned nthreads = block.x;
     lds().assign(ldsBytes + 16, 0);
     auto call = [&]() { kernel(args...); };
-    Sched s;
+    Sched& s = sched_storage();
     s.body = [](void* p) { (*static_cast<decltype(call)*>(p))(); };
     s.arg = &call;
     sched() = &s;
@@ -288,6 +306,11 @@ inline T __shfl_up(T v, unsigned delta, int width = 64) {
     return r;
 }
 
+inline float __uint_as_float(unsigned x) {
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz(unsigned(x)); }

@@ -14,7 +14,7 @@ import tempfile
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
-from tests.cases import CASES  # noqa: E402
+from tests.cases import CASES, case_args  # noqa: E402
 
 CARE_CPU = os.path.join(ROOT, "oracle", "_ref", "care-cpu")
 READSIM = os.path.join(ROOT, "care_amd", "_build", "readsim")
@@ -38,7 +38,7 @@ def main():
                             "--genome", str(case["genome"]), "--out", reads, "--truth", truth], check=True)
             odir = os.path.join(tmp, name + "_out")
             subprocess.run([CARE_CPU, "-i", reads, "-o", "ref.fastq", "-d", odir, "-t", "8", "-m", "8G"]
-                           + case["args"], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+                           + case_args(case, tmp), check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
             ev = subprocess.run([READSIM, "eval", "--orig", reads, "--corr", os.path.join(odir, "ref.fastq"),
                                  "--truth", truth], check=True, stdout=subprocess.PIPE, text=True)
             out[name] = {"input_md5": md5(reads), "output_md5": md5(os.path.join(odir, "ref.fastq")),

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from tests import helpers
-from tests.cases import CASES
+from tests.cases import CASES, case_args
 
 BASES = "ACGT"
 
@@ -40,7 +40,7 @@ def test_oracle_output_equals_reference(name, tmp_path):
     g = helpers.golden()[name]
     assert helpers.md5(reads) == g["input_md5"]
     p = subprocess.run([helpers.ORACLE_EXE, "-i", reads, "-o", "o.fastq", "-d", str(tmp_path / "out"), "-t", "8"]
-                       + case["args"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+                       + case_args(case, str(tmp_path)), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert p.returncode == 0, p.stdout
     assert helpers.md5(str(tmp_path / "out" / "o.fastq")) == g["output_md5"]
 
