@@ -74,7 +74,7 @@ def cpu_baseline(reads, w, sample, threads):
            "--fixedNumberOfReads", str(sample)] + w["args"]
     os.makedirs(outdir, exist_ok=True)
     p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    m = re.search(r"step2 ([0-9.]+) s -> ([0-9.]+) reads/s", p.stdout)
+    m = re.search(r"step2 ([0-9.]+) s step3 [0-9.]+ s -> ([0-9.]+) reads/s", p.stdout)
     if p.returncode != 0 or not m:
         log("bench: cpu baseline failed: %s" % p.stdout[-500:])
         return None
@@ -88,8 +88,8 @@ def cpu_baseline(reads, w, sample, threads):
     except OSError:
         pass
     return {"value": float(m.group(2)), "unit": "reads/s", "cores": threads, "kind": "port",
-            "sample": "%d anchors of the same read set (tables over all reads), step2 %.2f s, %s" % (
-                sample, float(m.group(1)), model)}
+            "sample": "first %d anchors of the same read set (tables built over all reads, not timed), "
+                      "query+correction %.2f s on %d threads, %s" % (sample, float(m.group(1)), threads, model)}
 
 
 def main():
@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="repeat-pe", choices=sorted(WORKLOADS))
     ap.add_argument("--cache", default=os.path.join("/tmp", "care_amd_bench"))
-    ap.add_argument("--cpu-sample", type=int, default=60000)
+    ap.add_argument("--cpu-sample", type=int, default=400000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=0)
@@ -132,6 +132,7 @@ def main():
     step_ms = []
     corr_ms = []
     query_ms = []
+    launches = 1
     cands = 0
     status = None
     for i in range(args.steps):
@@ -143,6 +144,7 @@ def main():
         corr_ms.append(st.correct_ms)
         query_ms.append(st.query_ms)
         cands = st.candidates
+        launches = max(1, st.launches)
         log("bench: step %d %.1f ms (query %.1f ms, correct kernel %.1f ms)" % (i + 1, step_ms[-1], st.query_ms,
                                                                              st.correct_ms))
     ms = sum(step_ms) / len(step_ms)
@@ -155,7 +157,8 @@ def main():
     qbytes = rs.qpitch * 4 if w["qual_bits"] else 0
     alg_bytes = n * (pitch_bytes + rs.max_length + 1) + cands * (pitch_bytes + 4 + 16 + qbytes)
     kern_ms = sum(corr_ms) / len(corr_ms)
-    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    launch_ms = kern_ms / launches
+    achieved = (alg_bytes / launches) / (launch_ms / 1e3) / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -181,7 +184,8 @@ def main():
                    "corrected_anchors": int((status & 1).sum()) if status is not None else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_correct"},
+                     "kernel": "k_correct", "launches_per_step": launches,
+                     "bytes_per_launch": int(alg_bytes / launches), "avg_launch_ms": round(launch_ms, 4)},
         "cpu_baseline": cpu,
     }
     eng.close()

@@ -69,6 +69,7 @@ class BatchStats(ctypes.Structure):
         ("total_ms", ctypes.c_double),
         ("raw_hits", ctypes.c_uint64),
         ("candidates", ctypes.c_uint64),
+        ("launches", ctypes.c_uint32),
     ]
 
 

@@ -607,6 +607,7 @@ int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_resu
             HIP_TRY(hipEventElapsedTime(&t2, ctx->ev[1], ctx->ev[3]));
             qms += t1;
             cms += t2;
+            ctx->stats.launches++;
             ctx->stats.raw_hits += ctx->hU64.p[0];
             ctx->stats.candidates += ctx->hU64.p[1];
             for (int i = 0; i < nb; i++) {

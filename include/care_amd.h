@@ -139,6 +139,7 @@ typedef struct care_batch_stats {
     double total_ms;       /* whole call incl. copies */
     uint64_t raw_hits;     /* table values gathered before de-duplication */
     uint64_t candidates;   /* candidates after query filtering */
+    uint32_t launches;     /* correction kernel launches (device batches) */
 } care_batch_stats;
 int care_last_batch_stats(care_context* ctx, care_batch_stats* stats);
 
