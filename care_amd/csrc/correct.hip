@@ -32,6 +32,22 @@ __device__ __forceinline__ u32 fetch_bits(const u32* p, int pos) {
     return s == 0 ? x : (x << s) | (p[w + 1] >> (32 - s));
 }
 
+// 16-byte candidate records moved as one vector load / store
+union MetaBits {
+    CandMeta m;
+    uint4 v;
+};
+__device__ __forceinline__ CandMeta load_meta(const CandMeta* p) {
+    MetaBits b;
+    b.v = *reinterpret_cast<const uint4*>(p);
+    return b.m;
+}
+__device__ __forceinline__ void store_meta(CandMeta* p, const CandMeta& m) {
+    MetaBits b;
+    b.m = m;
+    *reinterpret_cast<uint4*>(p) = b.v;
+}
+
 // pack the even bits (0,2,..,30) of x into the low 16 bits
 __device__ __forceinline__ u32 even_bits(u32 x) {
     x &= 0x55555555u;
@@ -42,7 +58,11 @@ __device__ __forceinline__ u32 even_bits(u32 x) {
     return x;
 }
 
+constexpr int kMetaChunk = 256;  // candidate records staged in LDS per MSA pass
+
 struct Lds {
+    CandMeta* mst;  // [kMetaChunk] candidate records of the current MSA chunk
+    float* qw;      // [256] quality weight per quality character
     u32* aw;        // anchor 2-bit words [pitch]
     u32* ahi;       // anchor planes [hw + 2]
     u32* alo;
@@ -61,6 +81,7 @@ struct LdsDims {
     int pitch, hw, maxLen, cols, waves;
     __host__ __device__ size_t bytes() const {
         size_t b = 0;
+        b += sizeof(CandMeta) * kMetaChunk + sizeof(float) * 256;
         b += sizeof(u32) * pitch;
         b += 2 * sizeof(u32) * (hw + 2);
         b += (size_t(maxLen) + 3) / 4 * 4;
@@ -76,6 +97,8 @@ struct LdsDims {
 __device__ Lds carve(char* base, const LdsDims& d) {
     Lds l;
     char* p = base;
+    l.mst = reinterpret_cast<CandMeta*>(p); p += sizeof(CandMeta) * kMetaChunk;
+    l.qw = reinterpret_cast<float*>(p); p += sizeof(float) * 256;
     l.aw = reinterpret_cast<u32*>(p); p += sizeof(u32) * d.pitch;
     l.ahi = reinterpret_cast<u32*>(p); p += sizeof(u32) * (d.hw + 2);
     l.alo = reinterpret_cast<u32*>(p); p += sizeof(u32) * (d.hw + 2);
@@ -97,7 +120,10 @@ struct Msa {
 };
 
 // Builds the MSA of the anchor and candidates meta[0..m) into LDS
-// (src/msa.cpp:9-173).  All threads must call.
+// (src/msa.cpp:9-173).  One thread per column; per column the weights are
+// summed anchor first, then candidates in index order, as on the CPU.
+// Candidate records are staged through LDS in chunks of kMetaChunk.  All
+// threads must call.
 __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const CandMeta* meta, int m, int la, int* sMin,
                          int* sMax) {
     if (threadIdx.x == 0) {
@@ -121,39 +147,58 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const CandMeta* m
     const DevReadStore& rs = P.rs;
     const bool useQ = P.useQuality != 0;
     const int cols = msa.cols;
-    for (int col = threadIdx.x; col < cols; col += blockDim.x) {
+    for (int cb = 0; cb < cols; cb += blockDim.x) {
+        const int col = cb + int(threadIdx.x);
+        const bool active = col < cols;
         int cA = 0, cC = 0, cG = 0, cT = 0;
         float wA = 0.f, wC = 0.f, wG = 0.f, wT = 0.f;
-        auto add = [&](u32 b, float w) {
-            switch (b) {
-                case 0: cA++; wA += w; break;
-                case 1: cC++; wC += w; break;
-                case 2: cG++; wG += w; break;
-                default: cT++; wT += w; break;
-            }
-        };
+        // conditional adds of +0.0f leave the other sums bit-identical
+        // (no sum is ever -0.0f)
+#define CARE_MSA_ADD(b, w)                        \
+    do {                                          \
+        const u32 b_ = (b);                       \
+        const float w_ = (w);                     \
+        cA += b_ == 0u;                           \
+        cC += b_ == 1u;                           \
+        cG += b_ == 2u;                           \
+        cT += b_ == 3u;                           \
+        wA += b_ == 0u ? w_ : 0.0f;               \
+        wC += b_ == 1u ? w_ : 0.0f;               \
+        wG += b_ == 2u ? w_ : 0.0f;               \
+        wT += b_ == 3u ? w_ : 0.0f;               \
+    } while (0)
         u32 anchorBase = 0;
-        if (col >= msa.ab && col < ae) {
+        const bool inAnchor = active && col >= msa.ab && col < ae;
+        if (inAnchor) {
             const int i = col - msa.ab;
             anchorBase = base_at(L.aw, i);
-            const float w = useQ ? P.qweights[(unsigned char)L.anchorQ[i]] : 1.0f;
-            add(anchorBase, 1.0f * w);
+            const float w = useQ ? L.qw[(unsigned char)L.anchorQ[i]] : 1.0f;
+            CARE_MSA_ADD(anchorBase, 1.0f * w);
         }
-        for (int c = 0; c < m; c++) {
-            const CandMeta cm = meta[c];
-            const int pos = col - msa.ab - int(cm.shift);
-            if (pos < 0 || pos >= int(cm.len)) continue;
-            const bool rc = cm.flag == 2;
-            const u32* enc = rs.seqs + u64(cm.id) * rs.pitch;
-            const u32 b = oriented_base(enc, cm.len, pos, rc);
-            float w = cm.u.weight;
-            if (useQ) {
-                const int qi = rc ? rs.maxLen - 1 - pos : pos;
-                const char q = quality_char(rs.quals + u64(cm.id) * rs.qpitch, qi, rs.qbits);
-                w = cm.u.weight * P.qweights[(unsigned char)q];
+        for (int c0 = 0; c0 < m; c0 += kMetaChunk) {
+            const int nc = min(kMetaChunk, m - c0);
+            __syncthreads();
+            for (int i = threadIdx.x; i < nc; i += blockDim.x) store_meta(L.mst + i, load_meta(meta + c0 + i));
+            __syncthreads();
+            if (!active) continue;
+            for (int c = 0; c < nc; c++) {
+                const CandMeta cm = L.mst[c];
+                const int pos = col - msa.ab - int(cm.shift);
+                if (pos < 0 || pos >= int(cm.len)) continue;
+                const bool rc = cm.flag == 2;
+                const u32* enc = rs.seqs + u64(cm.id) * rs.pitch;
+                const u32 b = oriented_base(enc, cm.len, pos, rc);
+                float w = cm.u.weight;
+                if (useQ) {
+                    const int qi = rc ? rs.maxLen - 1 - pos : pos;
+                    const char q = quality_char(rs.quals + u64(cm.id) * rs.qpitch, qi, rs.qbits);
+                    w = cm.u.weight * L.qw[(unsigned char)q];
+                }
+                CARE_MSA_ADD(b, w);
             }
-            add(b, w);
         }
+#undef CARE_MSA_ADD
+        if (!active) continue;
         u8 cons = 0;
         float cw = wA;
         if (wC > cw) { cons = 1; cw = wC; }
@@ -171,17 +216,23 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const CandMeta* m
         L.wgt[1 * cols + col] = wC;
         L.wgt[2 * cols + col] = wG;
         L.wgt[3 * cols + col] = wT;
-        if (col >= msa.ab && col < ae) {
-            const int oc[4] = {cA, cC, cG, cT};
-            L.origCov[col] = oc[anchorBase];
-        }
+        if (inAnchor)
+            L.origCov[col] = anchorBase == 0u ? cA : anchorBase == 1u ? cC : anchorBase == 2u ? cG : cT;
     }
     __syncthreads();
     return msa;
 }
 
+// upper 32 bits of (hi:lo) << r, r in 0..31
+__device__ __forceinline__ u32 funnel(u32 hi, u32 lo, int r) {
+    return r ? __builtin_amdgcn_alignbit(hi, lo, u32(32 - r)) : hi;
+}
+
 // Alignment of the anchor against the candidate planes of one wave, all
 // shifts in parallel; result key = score << 11 | order index, or ~0u.
+// For shift s >= 0 the anchor is read from bit s against the candidate from
+// bit 0, for s < 0 the candidate from bit -s against the anchor from bit 0:
+// one side is always word aligned.  Planes are padded with two words.
 __device__ __forceinline__ void align_wave(const u32* ahi, const u32* alo, int la, const u32* fhi, const u32* flo,
                                            const u32* rhi, const u32* rlo, int lc, int minov, float maxErrorRate,
                                            u32* keyF, u32* keyR) {
@@ -198,17 +249,24 @@ __device__ __forceinline__ void align_wave(const u32* ahi, const u32* alo, int l
         const int ov = s >= 0 ? min(la - s, lc) : min(la, lc + s);
         const int lim = int(float(ov) * maxErrorRate);
         if (lim <= 0) continue;
-        const int sa = s > 0 ? s : 0;
-        const int sc = s < 0 ? -s : 0;
         const u32* chi = isRc ? rhi : fhi;
         const u32* clo = isRc ? rlo : flo;
+        const int d = s >= 0 ? s : -s;
+        const u32* uh = (s >= 0 ? ahi : chi) + (d >> 5);
+        const u32* ul = (s >= 0 ? alo : clo) + (d >> 5);
+        const u32* vh = s >= 0 ? chi : ahi;
+        const u32* vl = s >= 0 ? clo : alo;
+        const int r = d & 31;
+        u32 h0 = uh[0], l0 = ul[0];
         int mm = 0;
-        for (int off = 0; off < ov && mm < lim; off += 32) {
-            const int nb = ov - off < 32 ? ov - off : 32;
-            u32 x = (fetch_bits(ahi, sa + off) ^ fetch_bits(chi, sc + off)) |
-                    (fetch_bits(alo, sa + off) ^ fetch_bits(clo, sc + off));
+        for (int j = 0; 32 * j < ov && mm < lim; j++) {
+            const u32 h1 = uh[j + 1], l1 = ul[j + 1];
+            u32 x = (funnel(h0, h1, r) ^ vh[j]) | (funnel(l0, l1, r) ^ vl[j]);
+            const int nb = ov - 32 * j;
             if (nb < 32) x &= ~0u << (32 - nb);
             mm += __popc(x);
+            h0 = h1;
+            l0 = l1;
         }
         if (mm < lim) {
             const u32 key = (u32(mm + total - 2 * ov) << 11) | u32(u);
@@ -371,6 +429,9 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
     u32* rhi = myPlanes + 2 * hw2;
     u32* rlo = myPlanes + 3 * hw2;
 
+    if (P.useQuality) {
+        for (int i = tid; i < 256; i += blockDim.x) L.qw[i] = P.qweights[i];
+    }
     for (;;) {
         if (tid == 0) sAnchor = atomicAdd(P.queueHead, 1);
         __syncthreads();
@@ -553,7 +614,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             bool keep = false;
             CandMeta cm{};
             if (c < n) {
-                cm = metaA[c];
+                cm = load_meta(metaA + c);
                 if (cm.flag != 3) {
                     const float ratio = float(cm.u.al.nOps) / float(cm.u.al.overlap);
                     keep = P.pairedEnd ? (cm.paired || ratio < P.pairedThreshold) : (ratio < sThr);
@@ -564,7 +625,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             if (keep) {
                 const float w = 1.0f - sqrtf(float(cm.u.al.nOps) / (float(cm.u.al.overlap) * P.maxErrorRate));
                 cm.u.weight = w;
-                metaB[m + pos] = cm;
+                store_meta(metaB + m + pos, cm);
             }
             m += cnt;
         }
@@ -631,7 +692,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 bool keep = false;
                 CandMeta cm{};
                 if (c < m) {
-                    cm = cur[c];
+                    cm = load_meta(cur + c);
                     const int rb = msa.ab + cm.shift;
                     const int re = rb + cm.len;
                     keep = true;
@@ -642,7 +703,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 }
                 int cnt;
                 const int pos = block_exclusive_count(keep, L.scratch, &cnt);
-                if (keep) other[m2 + pos] = cm;
+                if (keep) store_meta(other + m2 + pos, cm);
                 m2 += cnt;
             }
             __syncthreads();

@@ -43,6 +43,9 @@ struct uint2 {
     unsigned x, y;
 };
 inline uint2 make_uint2(unsigned a, unsigned b) { return uint2{a, b}; }
+struct alignas(16) uint4 {
+    unsigned x, y, z, w;
+};
 
 typedef int hipError_t;
 typedef void* hipStream_t;
@@ -193,9 +196,29 @@ inline void run_block(Sched& s, const char* name) {
         makecontext(&f.uc, fiber_entry, 0);
         f.started = false;
     }
+    // EMU_SCHED=reverse|random: order in which runnable fibers are resumed
+    // in every pass (default: thread order), to vary the interleaving of
+    // waves between barriers.
+    static const int schedMode = [] {
+        const char* e = std::getenv("EMU_SCHED");
+        if (!e) return 0;
+        return std::strcmp(e, "reverse") == 0 ? 1 : std::strcmp(e, "random") == 0 ? 2 : 0;
+    }();
+    static unsigned long long rng = 0x9E3779B97F4A7C15ull;
+    std::vector<int> order(static_cast<size_t>(n));
+    for (int i = 0; i < n; i++) order[size_t(i)] = schedMode == 1 ? n - 1 - i : i;
     for (;;) {
         bool allDone = true, progress = false;
-        for (int i = 0; i < n; i++) {
+        if (schedMode == 2) {
+            for (int i = n - 1; i > 0; i--) {
+                rng ^= rng << 13;
+                rng ^= rng >> 7;
+                rng ^= rng << 17;
+                std::swap(order[size_t(i)], order[size_t(rng % unsigned(i + 1))]);
+            }
+        }
+        for (int k = 0; k < n; k++) {
+            const int i = order[size_t(k)];
             Fiber& f = s.fibers[i];
             if (f.done) continue;
             allDone = false;
@@ -230,6 +253,17 @@ inline Sched& sched_storage() {
     return s;
 }
 
+// EMU_POISON=<byte>: fill device allocations and, before every block, the
+// dynamic LDS with that byte instead of zeros, so reads of memory that no
+// kernel or copy wrote show up as changed results.
+inline int poison_byte() {
+    static const int v = [] {
+        const char* e = std::getenv("EMU_POISON");
+        return e ? int(std::strtol(e, nullptr, 0) & 0xFF) : -1;
+    }();
+    return v;
+}
+
 template <class K, class... Args>
 void launch_named(const char* name, K kernel, dim3 grid, dim3 block, size_t ldsBytes, Args... args) {
     if (block.x > block_cap()) block.x = block_cap();
@@ -242,6 +276,7 @@ void launch_named(const char* name, K kernel, dim3 grid, dim3 block, size_t ldsB
     sched() = &s;
     s.fibers.resize(nthreads);
     for (unsigned b = 0; b < grid.x; b++) {
+        if (poison_byte() >= 0) std::memset(lds().data(), poison_byte(), lds().size());
         for (auto& f : s.fibers) {
             f.done = false;
             f.waitKind = 0;
@@ -277,6 +312,9 @@ inline void __syncthreads() { emu::block_barrier(); }
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_wave_barrier() emu::wave_barrier()
 
+inline unsigned __builtin_amdgcn_alignbit(unsigned a, unsigned b, unsigned c) {
+    return unsigned((((unsigned long long)a << 32) | b) >> (c & 31));
+}
 inline unsigned __lane_id() { return unsigned(emu::lane()); }
 inline unsigned long long __ballot(int pred) {
     emu::Wave& w = emu::my_wave();
@@ -381,6 +419,7 @@ inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
 template <class T>
 inline hipError_t hipMalloc(T** p, size_t bytes) {
     *p = static_cast<T*>(std::calloc(1, bytes ? bytes : 1));
+    if (*p && emu::poison_byte() >= 0) std::memset(static_cast<void*>(*p), emu::poison_byte(), bytes);
     return *p ? hipSuccess : hipErrorInvalidValue;
 }
 inline hipError_t hipFree(void* p) {
