@@ -566,7 +566,7 @@ int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_resu
         ensure_batch(ctx, B);
         const int T = ctx->blockThreads;
         int blocksPerCU = 0;
-        const size_t lds = correct_lds_bytes(ctx->rs.maxLen, T);
+        const size_t lds = correct_lds_bytes(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T);
         blocksPerCU = int(std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds + 1024))));
         hipStream_t s = ctx->stream;
         P.candIds = ctx->candIds.p;

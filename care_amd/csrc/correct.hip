@@ -11,6 +11,8 @@
 // include/gpu/gpumsa.cuh:743-812).  Floating-point sums run in the CPU's
 // order: per column, anchor first, then candidates in ascending index.
 
+#include <algorithm>
+
 #include "device_common.h"
 
 namespace care_engine {
@@ -23,14 +25,6 @@ __device__ __forceinline__ bool feq(float l, float r) {
 }
 __device__ __forceinline__ bool fleq(float l, float r) { return (l < r) || feq(l, r); }
 __device__ __forceinline__ bool fgeq(float l, float r) { return (l > r) || feq(l, r); }
-
-// 32 bits of a MSB-first bit plane starting at bit `pos` (plane padded with
-// at least one zero word past its end)
-__device__ __forceinline__ u32 fetch_bits(const u32* p, int pos) {
-    const int w = pos >> 5, s = pos & 31;
-    const u32 x = p[w];
-    return s == 0 ? x : (x << s) | (p[w + 1] >> (32 - s));
-}
 
 // 16-byte candidate records moved as one vector load / store
 union MetaBits {
@@ -58,16 +52,20 @@ __device__ __forceinline__ u32 even_bits(u32 x) {
     return x;
 }
 
-constexpr int kMetaChunk = 256;  // candidate records staged in LDS per MSA pass
-
+// LDS of one workgroup.  A sequence is held as interleaved bit planes: word
+// k is (hi bits, lo bits) of bases 32k..32k+31, MSB first.  Planes carry one
+// leading and two trailing zero words: plane word i lives at index i + 1, so
+// an unaligned 32-bit window starting at base d is always
+// alignbit(p[e], p[e + 1]) with e = d / 32 + (d % 32 != 0).
 struct Lds {
-    CandMeta* mst;  // [kMetaChunk] candidate records of the current MSA chunk
+    CandMeta* mst;  // [chunk] candidate records of the current MSA chunk
+    u32* rows;      // [chunk][pitch] their 2-bit rows
+    u32* qrows;     // [chunk][qpitch] their binned quality rows
     float* qw;      // [256] quality weight per quality character
     u32* aw;        // anchor 2-bit words [pitch]
-    u32* ahi;       // anchor planes [hw + 2]
-    u32* alo;
+    uint2* ap;      // anchor planes [hw + 3]
     char* anchorQ;  // [maxLen]
-    u32* planes;    // per wave: 4 planes of [hw + 2]
+    uint2* planes;  // per wave: forward and reverse-complement planes, 2 x [hw + 3]
     u8* cons;       // [cols]
     float* support;
     int* coverage;
@@ -78,14 +76,16 @@ struct Lds {
 };
 
 struct LdsDims {
-    int pitch, hw, maxLen, cols, waves;
+    int pitch, qpitch, hw, maxLen, cols, waves, chunk;
+    __host__ __device__ int pw() const { return hw + 3; }
     __host__ __device__ size_t bytes() const {
         size_t b = 0;
-        b += sizeof(CandMeta) * kMetaChunk + sizeof(float) * 256;
+        b += size_t(chunk) * (sizeof(CandMeta) + sizeof(u32) * (pitch + qpitch));
+        b += sizeof(float) * 256;
         b += sizeof(u32) * pitch;
-        b += 2 * sizeof(u32) * (hw + 2);
+        b += sizeof(uint2) * pw();
         b += (size_t(maxLen) + 3) / 4 * 4;
-        b += size_t(waves) * 4 * sizeof(u32) * (hw + 2);
+        b += size_t(waves) * 2 * sizeof(uint2) * pw();
         b += (size_t(cols) + 3) / 4 * 4;
         b += sizeof(float) * cols + 2 * sizeof(int) * cols;
         b += 4 * sizeof(u16) * cols + 4 * sizeof(float) * cols;
@@ -97,12 +97,14 @@ struct LdsDims {
 __device__ Lds carve(char* base, const LdsDims& d) {
     Lds l;
     char* p = base;
-    l.mst = reinterpret_cast<CandMeta*>(p); p += sizeof(CandMeta) * kMetaChunk;
+    // 16-byte aligned base: 16- and 8-byte records first
+    l.mst = reinterpret_cast<CandMeta*>(p); p += sizeof(CandMeta) * d.chunk;
+    l.ap = reinterpret_cast<uint2*>(p); p += sizeof(uint2) * d.pw();
+    l.planes = reinterpret_cast<uint2*>(p); p += size_t(d.waves) * 2 * sizeof(uint2) * d.pw();
+    l.rows = reinterpret_cast<u32*>(p); p += sizeof(u32) * size_t(d.chunk) * d.pitch;
+    l.qrows = reinterpret_cast<u32*>(p); p += sizeof(u32) * size_t(d.chunk) * d.qpitch;
     l.qw = reinterpret_cast<float*>(p); p += sizeof(float) * 256;
     l.aw = reinterpret_cast<u32*>(p); p += sizeof(u32) * d.pitch;
-    l.ahi = reinterpret_cast<u32*>(p); p += sizeof(u32) * (d.hw + 2);
-    l.alo = reinterpret_cast<u32*>(p); p += sizeof(u32) * (d.hw + 2);
-    l.planes = reinterpret_cast<u32*>(p); p += size_t(d.waves) * 4 * sizeof(u32) * (d.hw + 2);
     l.support = reinterpret_cast<float*>(p); p += sizeof(float) * d.cols;
     l.coverage = reinterpret_cast<int*>(p); p += sizeof(int) * d.cols;
     l.origCov = reinterpret_cast<int*>(p); p += sizeof(int) * d.cols;
@@ -122,10 +124,10 @@ struct Msa {
 // Builds the MSA of the anchor and candidates meta[0..m) into LDS
 // (src/msa.cpp:9-173).  One thread per column; per column the weights are
 // summed anchor first, then candidates in index order, as on the CPU.
-// Candidate records are staged through LDS in chunks of kMetaChunk.  All
-// threads must call.
-__device__ Msa build_msa(const CorrectParams& P, const Lds& L, const CandMeta* meta, int m, int la, int* sMin,
-                         int* sMax) {
+// Candidate records, 2-bit rows and quality rows are staged through LDS in
+// chunks of dims.chunk candidates.  All threads must call.
+__device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& dims, const CandMeta* meta, int m,
+                         int la, int* sMin, int* sMax) {
     if (threadIdx.x == 0) {
         *sMin = 0;
         *sMax = la;
@@ -147,6 +149,12 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const CandMeta* m
     const DevReadStore& rs = P.rs;
     const bool useQ = P.useQuality != 0;
     const int cols = msa.cols;
+    const int pitch = rs.pitch, qpitch = useQ ? rs.qpitch : 0;
+    // staging assignment: thread -> (row r0 + k * rowsPerStep, word w)
+    const int rowsS = int(blockDim.x) / pitch;
+    const int r0S = int(threadIdx.x) / pitch, wS = int(threadIdx.x) - r0S * pitch;
+    const int rowsQ = qpitch ? int(blockDim.x) / qpitch : 0;
+    const int r0Q = qpitch ? int(threadIdx.x) / qpitch : 0, wQ = int(threadIdx.x) - r0Q * qpitch;
     for (int cb = 0; cb < cols; cb += blockDim.x) {
         const int col = cb + int(threadIdx.x);
         const bool active = col < cols;
@@ -175,10 +183,16 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const CandMeta* m
             const float w = useQ ? L.qw[(unsigned char)L.anchorQ[i]] : 1.0f;
             CARE_MSA_ADD(anchorBase, 1.0f * w);
         }
-        for (int c0 = 0; c0 < m; c0 += kMetaChunk) {
-            const int nc = min(kMetaChunk, m - c0);
+        for (int c0 = 0; c0 < m; c0 += dims.chunk) {
+            const int nc = min(dims.chunk, m - c0);
             __syncthreads();
             for (int i = threadIdx.x; i < nc; i += blockDim.x) store_meta(L.mst + i, load_meta(meta + c0 + i));
+            if (r0S < rowsS)
+                for (int c = r0S; c < nc; c += rowsS)
+                    L.rows[c * pitch + wS] = rs.seqs[u64(meta[c0 + c].id) * pitch + wS];
+            if (r0Q < rowsQ)
+                for (int c = r0Q; c < nc; c += rowsQ)
+                    L.qrows[c * qpitch + wQ] = rs.quals[u64(meta[c0 + c].id) * qpitch + wQ];
             __syncthreads();
             if (!active) continue;
             for (int c = 0; c < nc; c++) {
@@ -186,12 +200,11 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const CandMeta* m
                 const int pos = col - msa.ab - int(cm.shift);
                 if (pos < 0 || pos >= int(cm.len)) continue;
                 const bool rc = cm.flag == 2;
-                const u32* enc = rs.seqs + u64(cm.id) * rs.pitch;
-                const u32 b = oriented_base(enc, cm.len, pos, rc);
+                const u32 b = oriented_base(L.rows + c * pitch, cm.len, pos, rc);
                 float w = cm.u.weight;
                 if (useQ) {
                     const int qi = rc ? rs.maxLen - 1 - pos : pos;
-                    const char q = quality_char(rs.quals + u64(cm.id) * rs.qpitch, qi, rs.qbits);
+                    const char q = quality_char(L.qrows + c * qpitch, qi, rs.qbits);
                     w = cm.u.weight * L.qw[(unsigned char)q];
                 }
                 CARE_MSA_ADD(b, w);
@@ -223,52 +236,49 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const CandMeta* m
     return msa;
 }
 
-// upper 32 bits of (hi:lo) << r, r in 0..31
-__device__ __forceinline__ u32 funnel(u32 hi, u32 lo, int r) {
-    return r ? __builtin_amdgcn_alignbit(hi, lo, u32(32 - r)) : hi;
-}
-
 // Alignment of the anchor against the candidate planes of one wave, all
 // shifts in parallel; result key = score << 11 | order index, or ~0u.
-// For shift s >= 0 the anchor is read from bit s against the candidate from
-// bit 0, for s < 0 the candidate from bit -s against the anchor from bit 0:
-// one side is always word aligned.  Planes are padded with two words.
-__device__ __forceinline__ void align_wave(const u32* ahi, const u32* alo, int la, const u32* fhi, const u32* flo,
-                                           const u32* rhi, const u32* rlo, int lc, int minov, float maxErrorRate,
-                                           u32* keyF, u32* keyR) {
+// For shift s >= 0 the anchor is read from base s against the candidate
+// from base 0, for s < 0 the candidate from base -s against the anchor from
+// base 0: one side is always word aligned.  The word loop is uniform over
+// the wave and ends when no lane is still below its mismatch limit; reads
+// past a lane's last word stay inside the workgroup's LDS and are masked.
+__device__ __forceinline__ void align_wave(const uint2* ap, int la, const uint2* fp, const uint2* rp, int lc,
+                                           int minov, float maxErrorRate, u32* keyF, u32* keyR) {
     const int lane = lane_id();
     const int nPos = max(0, la - minov + 1);
     const int nNeg = max(0, lc - minov);
     const int tot = nPos + nNeg;
     const int total = la + lc;
     u32 kf = ~0u, kr = ~0u;
-    for (int t = lane; t < 2 * tot; t += kWave) {
+    for (int t0 = 0; t0 < 2 * tot; t0 += kWave) {
+        const int t = t0 + lane;
+        const bool inRange = t < 2 * tot;
         const bool isRc = t >= tot;
-        const int u = isRc ? t - tot : t;
+        const int u = inRange ? (isRc ? t - tot : t) : 0;
         const int s = u < nPos ? u : -(u - nPos + 1);
         const int ov = s >= 0 ? min(la - s, lc) : min(la, lc + s);
         const int lim = int(float(ov) * maxErrorRate);
-        if (lim <= 0) continue;
-        const u32* chi = isRc ? rhi : fhi;
-        const u32* clo = isRc ? rlo : flo;
+        const int nw = (inRange && lim > 0) ? (ov + 31) >> 5 : 0;
+        const uint2* cp = isRc ? rp : fp;
         const int d = s >= 0 ? s : -s;
-        const u32* uh = (s >= 0 ? ahi : chi) + (d >> 5);
-        const u32* ul = (s >= 0 ? alo : clo) + (d >> 5);
-        const u32* vh = s >= 0 ? chi : ahi;
-        const u32* vl = s >= 0 ? clo : alo;
-        const int r = d & 31;
-        u32 h0 = uh[0], l0 = ul[0];
+        const u32 sh = u32(32 - (d & 31)) & 31u;
+        const uint2* up = (s >= 0 ? ap : cp) + (d >> 5) + ((d & 31) != 0);
+        const uint2* vp = (s >= 0 ? cp : ap) + 1;
+        const u32 lastMask = (ov & 31) ? ~(~0u >> (ov & 31)) : ~0u;
+        uint2 w0 = up[0];
         int mm = 0;
-        for (int j = 0; 32 * j < ov && mm < lim; j++) {
-            const u32 h1 = uh[j + 1], l1 = ul[j + 1];
-            u32 x = (funnel(h0, h1, r) ^ vh[j]) | (funnel(l0, l1, r) ^ vl[j]);
-            const int nb = ov - 32 * j;
-            if (nb < 32) x &= ~0u << (32 - nb);
-            mm += __popc(x);
-            h0 = h1;
-            l0 = l1;
+        for (int j = 0; __any(j < nw && mm < lim); j++) {
+            const uint2 w1 = up[j + 1];
+            const uint2 v = vp[j];
+            const u32 xh = __builtin_amdgcn_alignbit(w0.x, w1.x, sh) ^ v.x;
+            const u32 xl = __builtin_amdgcn_alignbit(w0.y, w1.y, sh) ^ v.y;
+            u32 msk = j + 1 < nw ? ~0u : lastMask;
+            msk = j < nw ? msk : 0u;
+            mm += __popc((xh | xl) & msk);
+            w0 = w1;
         }
-        if (mm < lim) {
+        if (nw > 0 && mm < lim) {
             const u32 key = (u32(mm + total - 2 * ov) << 11) | u32(u);
             if (isRc) kr = key < kr ? key : kr;
             else kf = key < kf ? key : kf;
@@ -413,8 +423,8 @@ __device__ void column_features(const Lds& L, int cols, int pos, u32 origBase, c
 }  // namespace
 
 __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) {
-    HIP_DYNAMIC_SHARED(char, ldsRaw)
-    const Lds L = carve(ldsRaw, dims);
+    HIP_DYNAMIC_SHARED(uint4, ldsRaw)  // uint4: 16-byte aligned dynamic LDS
+    const Lds L = carve(reinterpret_cast<char*>(ldsRaw), dims);
     __shared__ int sAnchor, sMin, sMax, sCount[3], sFlag, sCol, sTotal;
     __shared__ float sThr;
     const DevReadStore& rs = P.rs;
@@ -422,12 +432,9 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
     const int lane = lane_id();
     const int wave = tid / kWave;
     const int nwaves = blockDim.x / kWave;
-    const int hw2 = dims.hw + 2;
-    u32* myPlanes = L.planes + size_t(wave) * 4 * hw2;
-    u32* fhi = myPlanes;
-    u32* flo = myPlanes + hw2;
-    u32* rhi = myPlanes + 2 * hw2;
-    u32* rlo = myPlanes + 3 * hw2;
+    const int pw = dims.pw();
+    uint2* fp = L.planes + size_t(wave) * 2 * pw;
+    uint2* rp = fp + pw;
 
     if (P.useQuality) {
         for (int i = tid; i < 256; i += blockDim.x) L.qw[i] = P.qweights[i];
@@ -459,9 +466,10 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             for (int i = tid; i < la; i += blockDim.x) L.anchorQ[i] = quality_char(q, i, rs.qbits);
         }
         __syncthreads();
-        for (int w = tid; w < hw2; w += blockDim.x) {
+        for (int p = tid; p < pw; p += blockDim.x) {
+            const int w = p - 1;  // plane word (index 0 is padding)
             u32 hi = 0, lo = 0;
-            if (2 * w < rs.pitch) {
+            if (w >= 0 && 2 * w < rs.pitch) {
                 const u32 x0 = L.aw[2 * w];
                 const u32 x1 = 2 * w + 1 < rs.pitch ? L.aw[2 * w + 1] : 0u;
                 hi = (even_bits(x0 >> 1) << 16) | even_bits(x1 >> 1);
@@ -475,8 +483,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     lo &= mask;
                 }
             }
-            L.ahi[w] = hi;
-            L.alo[w] = lo;
+            L.ap[p] = make_uint2(hi, lo);
         }
         __syncthreads();
 
@@ -498,9 +505,10 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             const int lc = rs.lens[cid];
             const u32* cenc = rs.seqs + u64(cid) * rs.pitch;
             const int cw = (lc + 31) / 32;
-            for (int w = lane; w < hw2; w += kWave) {
+            for (int p = lane; p < pw; p += kWave) {
+                const int w = p - 1;
                 u32 hi = 0, lo = 0;
-                if (w < cw) {
+                if (w >= 0 && w < cw) {
                     const u32 x0 = cenc[2 * w];
                     const u32 x1 = 2 * w + 1 < rs.pitch ? cenc[2 * w + 1] : 0u;
                     hi = (even_bits(x0 >> 1) << 16) | even_bits(x1 >> 1);
@@ -512,22 +520,24 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                         lo &= mask;
                     }
                 }
-                fhi[w] = hi;
-                flo[w] = lo;
+                fp[p] = make_uint2(hi, lo);
             }
             wave_sync();
             // reverse complement planes: rc[q] = 1 - f[lc-1-q]
-            for (int w = lane; w < hw2; w += kWave) {
+            for (int p = lane; p < pw; p += kWave) {
+                const int w = p - 1;
                 u32 hi = 0, lo = 0;
-                if (w < cw) {
+                if (w >= 0 && w < cw) {
                     const int e = lc - 32 * (w + 1);  // first fwd bit of the window
                     u32 xh, xl;
                     if (e >= 0) {
-                        xh = fetch_bits(fhi, e);
-                        xl = fetch_bits(flo, e);
+                        const int k = e >> 5, r = e & 31;
+                        const uint2 a0 = fp[1 + k], a1 = fp[2 + k];
+                        xh = r ? (a0.x << r) | (a1.x >> (32 - r)) : a0.x;
+                        xl = r ? (a0.y << r) | (a1.y >> (32 - r)) : a0.y;
                     } else {
-                        xh = fhi[0] >> (-e);
-                        xl = flo[0] >> (-e);
+                        xh = fp[1].x >> (-e);
+                        xl = fp[1].y >> (-e);
                     }
                     hi = ~__brev(xh);
                     lo = ~__brev(xl);
@@ -538,12 +548,11 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                         lo &= mask;
                     }
                 }
-                rhi[w] = hi;
-                rlo[w] = lo;
+                rp[p] = make_uint2(hi, lo);
             }
             wave_sync();
             u32 kf, kr;
-            align_wave(L.ahi, L.alo, la, fhi, flo, rhi, rlo, lc, minovA, P.maxErrorRate, &kf, &kr);
+            align_wave(L.ap, la, fp, rp, lc, minovA, P.maxErrorRate, &kf, &kr);
             if (lane == 0) {
                 const Aln f = decode_key(kf, la, lc, minovA);
                 const Aln r = decode_key(kr, la, lc, minovA);
@@ -638,7 +647,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         // ---- phase C: MSA + refinement (include/corrector.hpp:1307-1421) ----
         CandMeta* cur = metaB;
         CandMeta* other = metaA;
-        Msa msa = build_msa(P, L, cur, m, la, &sMin, &sMax);
+        Msa msa = build_msa(P, L, dims, cur, m, la, &sMin, &sMax);
         for (int iter = 0; iter < 5 && m > 0; iter++) {
             // src/msa.cpp:532-734
             if (tid == 0) {
@@ -711,7 +720,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             cur = other;
             other = t;
             m = m2;
-            msa = build_msa(P, L, cur, m, la, &sMin, &sMax);
+            msa = build_msa(P, L, dims, cur, m, la, &sMin, &sMax);
         }
 
         // ---- phase D: anchor correction (src/msa.cpp:304-453, include/corrector.hpp:1423-1472) ----
@@ -827,25 +836,28 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
     }
 }
 
-size_t correct_lds_bytes(int maxLen, int threads) {
+// LDS shape of k_correct; MSA chunks hold up to 256 candidates and about
+// 16 KiB of staged rows
+static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads) {
     LdsDims d;
     d.pitch = (maxLen + 15) / 16;
+    d.qpitch = useQuality ? qpitch : 0;
     d.hw = (maxLen + 31) / 32;
     d.maxLen = maxLen;
     d.cols = 3 * maxLen + 2;
     d.waves = threads / kWave;
-    return d.bytes();
+    const int rowBytes = int(sizeof(CandMeta)) + 4 * (d.pitch + d.qpitch);
+    d.chunk = std::max(16, std::min(256, 16384 / rowBytes));
+    return d;
+}
+
+size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads) {
+    return make_dims(maxLen, useQuality, qpitch, threads).bytes();
 }
 
 void launch_correct(const CorrectParams& p, int blocks, int threads, hipStream_t stream) {
-    LdsDims d;
-    d.pitch = p.rs.pitch;
-    d.hw = (p.rs.maxLen + 31) / 32;
-    d.maxLen = p.rs.maxLen;
-    d.cols = 3 * p.rs.maxLen + 2;
-    d.waves = threads / kWave;
-    const size_t lds = d.bytes();
-    hipLaunchKernelGGL(k_correct, dim3(blocks), dim3(threads), lds, stream, p, d);
+    const LdsDims d = make_dims(p.rs.maxLen, p.useQuality, p.rs.qpitch, threads);
+    hipLaunchKernelGGL(k_correct, dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
 }
 
 }  // namespace care_engine

@@ -137,7 +137,7 @@ void launch_build_map(const DevReadStore& rs, const u64* sigs, int h, int map, i
 void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, int maxCand, int excludeAmbiguous,
                   u32* candIds, int* candCount, unsigned long long* rawHits, hipStream_t stream);
 void launch_correct(const CorrectParams& p, int blocks, int threads, hipStream_t stream);
-size_t correct_lds_bytes(int maxLen, int threads);
+size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads);
 int query_lds_bytes(int h, int cap, int maxLen, int threads);
 
 }  // namespace care_engine

@@ -77,8 +77,11 @@ struct Fiber {
     unsigned long waitGen = 0;
 };
 
+// Lane values of a wave operation go to buffer (generation & 1): a fiber
+// can only write the same buffer again two wave barriers later, after every
+// lane has read it, so one barrier per operation suffices.
 struct Wave {
-    unsigned long long vals[kWaveSize];
+    unsigned long long vals[2][kWaveSize];
     int arrived = 0;
     int size = 0;
     unsigned long gen = 0;
@@ -161,11 +164,10 @@ inline T from_bits(unsigned long long b) {
 template <class T>
 inline T exchange(T v, int src) {
     Wave& w = my_wave();
-    w.vals[lane()] = to_bits(v);
+    const unsigned long g = w.gen;
+    w.vals[g & 1][lane()] = to_bits(v);
     wave_barrier();
-    const T r = from_bits<T>(w.vals[src]);
-    wave_barrier();
-    return r;
+    return from_bits<T>(w.vals[g & 1][src]);
 }
 
 inline void fiber_entry() {
@@ -318,13 +320,19 @@ inline unsigned __builtin_amdgcn_alignbit(unsigned a, unsigned b, unsigned c) {
 inline unsigned __lane_id() { return unsigned(emu::lane()); }
 inline unsigned long long __ballot(int pred) {
     emu::Wave& w = emu::my_wave();
-    w.vals[emu::lane()] = pred ? 1ull : 0ull;
+    const unsigned long g = w.gen;
+    w.vals[g & 1][emu::lane()] = pred ? 1ull : 0ull;
     emu::wave_barrier();
     unsigned long long m = 0;
     for (int i = 0; i < w.size; i++)
-        if (w.vals[i]) m |= 1ull << i;
-    emu::wave_barrier();
+        if (w.vals[g & 1][i]) m |= 1ull << i;
     return m;
+}
+inline int __any(int pred) { return __ballot(pred) != 0ull; }
+inline int __all(int pred) {
+    emu::Wave& w = emu::my_wave();
+    const unsigned long long full = w.size >= 64 ? ~0ull : ((1ull << w.size) - 1);
+    return __ballot(pred) == full;
 }
 template <class T>
 inline T __shfl(T v, int src, int width = 64) {

@@ -20,14 +20,14 @@ EMU_DIR = os.path.join(helpers.ROOT, "tests", "emu")
 EMU_EXE = os.path.join(EMU_DIR, "_build", "care-gpu-emu")
 
 SMALL = {
-    "se_small": {"profile": "uniform-se", "seed": 201, "genome": 6000,
+    "se_small": {"profile": "uniform-se", "seed": 201, "genome": 1500,
                  "args": ["-c", "30", "--pairmode", "SE", "-k", "20", "-h", "16"]},
-    "pe_small_q2": {"profile": "repeat-pe", "seed": 202, "genome": 12000,
+    "pe_small_q2": {"profile": "repeat-pe", "seed": 202, "genome": 2000,
                     "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "2", "-k", "20", "-h", "32"]},
-    "se_mixed_small": {"profile": "mixed-se", "seed": 203, "genome": 8000,
+    "se_mixed_small": {"profile": "mixed-se", "seed": 203, "genome": 2000,
                        "args": ["-c", "50", "--pairmode", "SE", "--excludeAmbiguous", "-q", "-k", "20", "-h", "48",
                                 "--candidateCorrection"]},
-    "pe_forest_small": {"profile": "repeat-pe", "seed": 204, "genome": 8000,
+    "pe_forest_small": {"profile": "repeat-pe", "seed": 204, "genome": 1500,
                         "args": ["-c", "30", "--pairmode", "PE", "-q", "-k", "20", "-h", "32",
                                  "--candidateCorrection", "--correctionType", "1", "--correctionTypeCands", "1"],
                         "forest": True},
@@ -76,8 +76,7 @@ def test_emulated_kernels_match_oracle(emu_exe, name, tmp_path):
     assert _run(emu_exe, case, str(tmp_path)) == want
 
 
-@pytest.mark.parametrize("env", [{"EMU_POISON": "0xA5", "EMU_SCHED": "reverse"},
-                                 {"EMU_POISON": "0x3F", "EMU_SCHED": "random"}])
+@pytest.mark.parametrize("env", [{"EMU_POISON": "0xA5", "EMU_SCHED": "random"}])
 def test_emulated_kernels_poisoned_memory_and_wave_order(emu_exe, env, tmp_path):
     case = SMALL["se_mixed_small"]
     want = _oracle(case, str(tmp_path))
