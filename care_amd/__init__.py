@@ -47,6 +47,7 @@ class CareOptions(ctypes.Structure):
         ("correction_type_cands", ctypes.c_int),
         ("threshold_anchor", ctypes.c_float),
         ("threshold_cands", ctypes.c_float),
+        ("candidate_limit", ctypes.c_int),
     ]
 
 
@@ -73,6 +74,22 @@ class BatchStats(ctypes.Structure):
     ]
 
 
+MLP_FEATURES = 32
+MLP_PARAMS = 4737
+
+
+class MlpTrainOptions(ctypes.Structure):
+    _fields_ = [
+        ("epochs", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("lr", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
 # exported C symbols and their signatures
 ENGINE_SYMBOLS = {
     "care_options_default": (None, [ctypes.POINTER(CareOptions)]),
@@ -93,6 +110,16 @@ ENGINE_SYMBOLS = {
     "care_set_launch_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "care_forest_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint32]),
     "care_last_batch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchStats)]),
+    "care_mlp_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "care_mlp_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "care_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                      ctypes.POINTER(MlpTrainOptions), ctypes.c_void_p, ctypes.c_void_p]),
+    "care_mlp_infer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                      ctypes.c_void_p]),
+    "care_gate_columns": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 HOST_SYMBOLS = {
@@ -215,14 +242,20 @@ class ReadSet:
 class Engine:
     """One device context of the HIP engine."""
 
-    def __init__(self, options, device=0):
-        self.lib = engine()
+    def __init__(self, options, device=0, lib=None):
+        """lib: an already bound engine library (default: the built HIP
+        engine)."""
+        self.lib = lib if lib is not None else engine()
         self.options = options
         ctx = ctypes.c_void_p()
-        _check(self.lib.care_context_create(device, ctypes.byref(options), ctypes.byref(ctx)))
+        self._ck(self.lib.care_context_create(device, ctypes.byref(options), ctypes.byref(ctx)))
         self.ctx = ctx
         self.n = 0
         self.max_length = 0
+
+    def _ck(self, rc):
+        if rc != CARE_OK:
+            raise CareError("care_amd error %d: %s" % (rc, self.lib.care_last_error().decode()))
 
     def close(self):
         if self.ctx:
@@ -242,7 +275,7 @@ class Engine:
         amb = None if ambiguous is None else np.ascontiguousarray(ambiguous, dtype=np.uint8)
         q = None if quals is None else np.ascontiguousarray(quals, dtype=np.uint32)
         ml = int(max_length if max_length is not None else lengths.max())
-        _check(self.lib.care_readstore_upload(
+        self._ck(self.lib.care_readstore_upload(
             self.ctx, n, ml, seqs.ctypes.data, pitch, lengths.ctypes.data,
             None if amb is None else amb.ctypes.data, None if q is None else q.ctypes.data,
             0 if q is None else q.shape[1], qual_bits))
@@ -254,14 +287,14 @@ class Engine:
         self.upload(rs.seqs, rs.lengths, rs.ambiguous, quals, rs.qual_bits or 8, rs.max_length)
 
     def build(self):
-        _check(self.lib.care_minhash_build(self.ctx))
+        self._ck(self.lib.care_minhash_build(self.ctx))
 
     def set_launch_config(self, batch=0, threads=0):
-        _check(self.lib.care_set_launch_config(self.ctx, batch, threads))
+        self._ck(self.lib.care_set_launch_config(self.ctx, batch, threads))
 
     def signatures(self, first, n):
         out = np.zeros((n, self.options.num_hash_maps), dtype=np.uint64)
-        _check(self.lib.care_minhash_signatures(self.ctx, first, n, out.ctypes.data))
+        self._ck(self.lib.care_minhash_signatures(self.ctx, first, n, out.ctypes.data))
         return out
 
     def query(self, first, n, capacity=None):
@@ -273,7 +306,7 @@ class Engine:
             if rc == 4 and capacity is None:
                 cap *= 4
                 continue
-            _check(rc)
+            self._ck(rc)
             return ids[: offs[n]], offs
 
     def correct(self, first, n, candidate_capacity=None):
@@ -290,14 +323,68 @@ class Engine:
         r.candidate_read_ids = cids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
         r.candidate_sequences = cseqs.ctypes.data_as(ctypes.POINTER(ctypes.c_char))
         r.candidate_capacity = ccap
-        _check(self.lib.care_correct_batch(self.ctx, first, n, ctypes.byref(r)))
+        self._ck(self.lib.care_correct_batch(self.ctx, first, n, ctypes.byref(r)))
         nc = r.num_candidates
         return status, seqs, cids[:nc], cseqs[:nc]
 
     def stats(self):
         s = BatchStats()
-        _check(self.lib.care_last_batch_stats(self.ctx, ctypes.byref(s)))
+        self._ck(self.lib.care_last_batch_stats(self.ctx, ctypes.byref(s)))
         return s
+
+    # ---- learned gate ----
+    def mlp_load(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.shape == (MLP_PARAMS,)
+        self._ck(self.lib.care_mlp_load(self.ctx, p.ctypes.data))
+
+    def mlp_gradient(self, features, labels, params):
+        x = np.ascontiguousarray(features, dtype=np.float32)
+        y = np.ascontiguousarray(labels, dtype=np.uint8)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._ck(self.lib.care_mlp_gradient(self.ctx, x.ctypes.data, y.ctypes.data, len(y), p.ctypes.data,
+                                          g.ctypes.data, ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_train(self, features, labels, params, epochs=5, batch=4096, lr=1e-3, beta1=0.9, beta2=0.999,
+                  eps=1e-8, seed=1):
+        x = np.ascontiguousarray(features, dtype=np.float32)
+        y = np.ascontiguousarray(labels, dtype=np.uint8)
+        p = np.array(params, dtype=np.float32)
+        o = MlpTrainOptions(epochs, batch, lr, beta1, beta2, eps, seed)
+        losses = np.zeros(epochs, dtype=np.float32)
+        self._ck(self.lib.care_mlp_train(self.ctx, x.ctypes.data, y.ctypes.data, len(y), ctypes.byref(o),
+                                       p.ctypes.data, losses.ctypes.data))
+        return p, losses
+
+    def mlp_infer(self, features, params):
+        x = np.ascontiguousarray(features, dtype=np.float32)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        z = np.zeros(len(x), dtype=np.float32)
+        self._ck(self.lib.care_mlp_infer(self.ctx, x.ctypes.data, len(x), p.ctypes.data, z.ctypes.data))
+        return z
+
+    def gate_columns(self, first, n, capacity=None):
+        """Decision columns of the gated mode: features [c][32], read ids,
+        positions, original and consensus bases."""
+        cap = capacity if capacity is not None else max(1, n) * 8
+        while True:
+            f = np.zeros((cap, MLP_FEATURES), dtype=np.float32)
+            ids = np.zeros(cap, dtype=np.uint32)
+            pos = np.zeros(cap, dtype=np.uint16)
+            orig = np.zeros(cap, dtype=np.uint8)
+            cons = np.zeros(cap, dtype=np.uint8)
+            cnt = ctypes.c_uint64()
+            rc = self.lib.care_gate_columns(self.ctx, first, n, f.ctypes.data, ids.ctypes.data, pos.ctypes.data,
+                                            orig.ctypes.data, cons.ctypes.data, cap, ctypes.byref(cnt))
+            if rc == 4 and capacity is None:
+                cap *= 4
+                continue
+            self._ck(rc)
+            c = cnt.value
+            return f[:c], ids[:c], pos[:c], orig[:c], cons[:c]
 
 
 def run_care_gpu(args, **kw):

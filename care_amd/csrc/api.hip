@@ -163,6 +163,22 @@ struct care_context {
         DevForest dev;
         bool loaded = false;
     } forest[2];
+
+    // learned gate: FP32 parameters and BF16 weight copies used by the
+    // correction kernel, plus training state
+    struct GateStore {
+        DevBuf<float> params;
+        DevBuf<unsigned short> w1, w2, w3;
+        bool loaded = false;
+    } gate;
+    struct GateRecord {
+        DevBuf<float> feat;
+        DevBuf<u32> ids;
+        DevBuf<unsigned short> pos;
+        DevBuf<u8> orig, cons;
+        int cap = 0;
+        bool on = false;
+    } record;
 };
 
 namespace {
@@ -237,6 +253,7 @@ void care_options_default(care_options* o) {
     o->correction_type_cands = 0;
     o->threshold_anchor = 0.5f;
     o->threshold_cands = 0.5f;
+    o->candidate_limit = 0;
 }
 
 const char* care_last_error(void) { return g_lastError.c_str(); }
@@ -248,9 +265,10 @@ int care_context_create(int device, const care_options* opt, care_context** out)
         return fail(CARE_ERR_ARGUMENT, "kmer_length %d outside 1..31", opt->kmer_length);
     if (opt->num_hash_maps < 1 || opt->num_hash_maps > 64)
         return fail(CARE_ERR_ARGUMENT, "num_hash_maps %d outside 1..64", opt->num_hash_maps);
-    if ((opt->correction_type != 0 && opt->correction_type != 1) ||
+    if ((opt->correction_type != 0 && opt->correction_type != 1 && opt->correction_type != 3) ||
         (opt->correction_type_cands != 0 && opt->correction_type_cands != 1))
-        return fail(CARE_ERR_ARGUMENT, "correction types: 0 classic, 1 forest");
+        return fail(CARE_ERR_ARGUMENT, "correction types: 0 classic, 1 forest, 3 learned gate (anchors only)");
+    if (opt->candidate_limit < 0) return fail(CARE_ERR_ARGUMENT, "candidate_limit must be >= 0");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CARE_ERR_DEVICE, "no HIP device available");
     if (device < 0 || device >= count) return fail(CARE_ERR_ARGUMENT, "device %d outside 0..%d", device, count - 1);
@@ -491,6 +509,7 @@ int care_minhash_query(care_context* ctx, uint64_t first, int n, uint32_t* ids, 
         for (int done = 0; done < n; done += B) {
             const int nb = std::min(B, n - done);
             launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->opt.exclude_ambiguous,
+                         ctx->opt.correction_type == 3 ? ctx->opt.candidate_limit : 0,
                          ctx->candIds.p, ctx->candCount.p, nullptr, ctx->stream);
             HIP_TRY(hipGetLastError());
             counts.resize(nb);
@@ -556,6 +575,27 @@ int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_resu
         if (!ctx->forest[0].loaded) return fail(CARE_ERR_STATE, "correction_type 1 needs an anchor forest (care_forest_load)");
         P.forestA = ctx->forest[0].dev;
     }
+    if (o.correction_type == 3) {
+        if (!ctx->gate.loaded && !ctx->record.on)
+            return fail(CARE_ERR_STATE, "correction_type 3 needs gate weights (care_mlp_load)");
+        if (ctx->gate.loaded) {
+            P.gateW1 = ctx->gate.w1.p;
+            P.gateW2 = ctx->gate.w2.p;
+            P.gateW3 = ctx->gate.w3.p;
+            P.gateParams = ctx->gate.params.p;
+        }
+        const float t = std::min(std::max(o.threshold_anchor, 1e-6f), 1.0f - 1e-6f);
+        P.gateLogit = std::log(t / (1.0f - t));
+        if (ctx->record.on) {
+            P.gateFeat = ctx->record.feat.p;
+            P.gateIds = ctx->record.ids.p;
+            P.gatePos = ctx->record.pos.p;
+            P.gateOrig = ctx->record.orig.p;
+            P.gateCons = ctx->record.cons.p;
+            P.gateCount = ctx->counters.p + 2;
+            P.gateCap = ctx->record.cap;
+        }
+    }
     if (o.correct_candidates && o.correction_type_cands == 1) {
         if (!ctx->forest[1].loaded) return fail(CARE_ERR_STATE, "correction_type_cands 1 needs a candidate forest (care_forest_load)");
         P.forestC = ctx->forest[1].dev;
@@ -566,7 +606,7 @@ int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_resu
         ensure_batch(ctx, B);
         const int T = ctx->blockThreads;
         int blocksPerCU = 0;
-        const size_t lds = correct_lds_bytes(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T);
+        const size_t lds = correct_lds_bytes(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T, o.correction_type == 3);
         blocksPerCU = int(std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds + 1024))));
         hipStream_t s = ctx->stream;
         P.candIds = ctx->candIds.p;
@@ -587,7 +627,8 @@ int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_resu
             const int nb = std::min(B, n - done);
             launch_fill_u64(reinterpret_cast<u64*>(ctx->rawHits.p), 2, 0, s);
             HIP_TRY(hipEventRecord(ctx->ev[0], s));
-            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, o.exclude_ambiguous, ctx->candIds.p,
+            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, o.exclude_ambiguous,
+                         o.correction_type == 3 ? o.candidate_limit : 0, ctx->candIds.p,
                          ctx->candCount.p, ctx->rawHits.p, s);
             HIP_TRY(hipEventRecord(ctx->ev[1], s));
             launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);
@@ -761,6 +802,269 @@ int care_forest_load(care_context* ctx, int which, const char* path, uint32_t ma
     st.dev.numTrees = int(starts.size());
     st.dev.threshold = which == 0 ? ctx->opt.threshold_anchor : ctx->opt.threshold_cands;
     st.loaded = true;
+    return CARE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned gate: parameters, training, inference, recorded decision columns
+// ---------------------------------------------------------------------------
+
+int care_mlp_load(care_context* ctx, const float* params) {
+    if (!ctx || !params) return fail(CARE_ERR_ARGUMENT, "null argument");
+    for (int i = 0; i < CARE_MLP_PARAMS; i++)
+        if (!std::isfinite(params[i])) return fail(CARE_ERR_ARGUMENT, "parameter %d is not finite", i);
+    auto& g = ctx->gate;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        g.params.alloc(CARE_MLP_PARAMS);
+        g.w1.alloc(64 * 32);
+        g.w2.alloc(32 * 64);
+        g.w3.alloc(16 * 32);
+        HIP_TRY(hipMemcpy(g.params.p, params, sizeof(float) * CARE_MLP_PARAMS, hipMemcpyHostToDevice));
+        launch_mlp_cast(g.params.p, g.w1.p, g.w2.p, g.w3.p, ctx->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    g.loaded = true;
+    return CARE_OK;
+}
+
+namespace {
+
+// features rounded to bf16 on the host, exactly as the device rounds
+std::vector<unsigned short> to_bf16(const float* x, uint64_t n) {
+    std::vector<unsigned short> out(size_t(n) * CARE_MLP_FEATURES);
+    for (size_t i = 0; i < out.size(); i++) {
+        unsigned u;
+        std::memcpy(&u, &x[i], 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        out[i] = (unsigned short)(u >> 16);
+    }
+    return out;
+}
+
+bool all_finite(const float* x, size_t n) {
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(x[i])) return false;
+    return true;
+}
+
+}  // namespace
+
+int care_mlp_gradient(care_context* ctx, const float* features, const uint8_t* labels, int n, const float* params,
+                      float* grad_out, float* loss_out) {
+    if (!ctx || !features || !labels || !params || !grad_out) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (n < 1 || n > (1 << 20)) return fail(CARE_ERR_ARGUMENT, "n %d outside 1..2^20", n);
+    if (!all_finite(features, size_t(n) * CARE_MLP_FEATURES) || !all_finite(params, CARE_MLP_PARAMS))
+        return fail(CARE_ERR_ARGUMENT, "non-finite input");
+    for (int i = 0; i < n; i++)
+        if (labels[i] > 1) return fail(CARE_ERR_ARGUMENT, "label %d is not 0 or 1", i);
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        const std::vector<unsigned short> xb = to_bf16(features, uint64_t(n));
+        std::vector<u32> perm(static_cast<size_t>(n));
+        for (int i = 0; i < n; i++) perm[size_t(i)] = u32(i);
+        DevBuf<unsigned short> X, w1, w2, w3;
+        DevBuf<u8> Y;
+        DevBuf<u32> P;
+        DevBuf<float> master, partial, out;
+        X.alloc(xb.size());
+        Y.alloc(size_t(n));
+        P.alloc(size_t(n));
+        master.alloc(CARE_MLP_PARAMS);
+        w1.alloc(64 * 32);
+        w2.alloc(32 * 64);
+        w3.alloc(16 * 32);
+        const int np = mlp_partials(n);
+        partial.alloc(size_t(np) * mlp_partial_stride());
+        out.alloc(CARE_MLP_PARAMS + 1);
+        HIP_TRY(hipMemcpy(X.p, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(Y.p, labels, size_t(n), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(master.p, params, sizeof(float) * CARE_MLP_PARAMS, hipMemcpyHostToDevice));
+        launch_mlp_cast(master.p, w1.p, w2.p, w3.p, s);
+        launch_mlp_grad(X.p, Y.p, P.p, 0, n, w1.p, w2.p, w3.p, master.p, 1.0f / float(n), partial.p, s);
+        launch_mlp_reduce(partial.p, np, 0, out.p, nullptr, nullptr, nullptr, 0, 0, 0, 0, 1, 1, nullptr, nullptr,
+                          nullptr, nullptr, s);
+        HIP_TRY(hipGetLastError());
+        std::vector<float> res(CARE_MLP_PARAMS + 1);
+        HIP_TRY(hipMemcpyAsync(res.data(), out.p, res.size() * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        std::memcpy(grad_out, res.data(), sizeof(float) * CARE_MLP_PARAMS);
+        if (loss_out) *loss_out = res[CARE_MLP_PARAMS];
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_mlp_train(care_context* ctx, const float* features, const uint8_t* labels, uint64_t n,
+                   const care_mlp_train_options* o, float* params, float* epoch_loss) {
+    if (!ctx || !features || !labels || !o || !params) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (n < 1 || n >= 0xFFFFFFFFull) return fail(CARE_ERR_ARGUMENT, "n out of range");
+    if (o->epochs < 1 || o->batch < 16 || o->batch > (1 << 16) || !(o->lr > 0.f))
+        return fail(CARE_ERR_ARGUMENT, "invalid training options");
+    if (!all_finite(params, CARE_MLP_PARAMS)) return fail(CARE_ERR_ARGUMENT, "non-finite initial parameters");
+    if (!all_finite(features, size_t(n) * CARE_MLP_FEATURES)) return fail(CARE_ERR_ARGUMENT, "non-finite features");
+    for (uint64_t i = 0; i < n; i++)
+        if (labels[i] > 1) return fail(CARE_ERR_ARGUMENT, "label %llu is not 0 or 1", (unsigned long long)i);
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        const std::vector<unsigned short> xb = to_bf16(features, n);
+        DevBuf<unsigned short> X, w1, w2, w3;
+        DevBuf<u8> Y;
+        DevBuf<u32> P;
+        DevBuf<float> master, m1, m2, partial, loss;
+        X.alloc(xb.size());
+        Y.alloc(size_t(n));
+        P.alloc(size_t(n));
+        master.alloc(CARE_MLP_PARAMS);
+        m1.alloc(CARE_MLP_PARAMS);
+        m2.alloc(CARE_MLP_PARAMS);
+        w1.alloc(64 * 32);
+        w2.alloc(32 * 64);
+        w3.alloc(16 * 32);
+        const int np = mlp_partials(o->batch);
+        partial.alloc(size_t(np) * mlp_partial_stride());
+        loss.alloc(1);
+        HIP_TRY(hipMemcpy(X.p, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(Y.p, labels, size_t(n), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(master.p, params, sizeof(float) * CARE_MLP_PARAMS, hipMemcpyHostToDevice));
+        std::vector<float> zeros(CARE_MLP_PARAMS, 0.f);
+        HIP_TRY(hipMemcpy(m1.p, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(m2.p, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice));
+        launch_mlp_cast(master.p, w1.p, w2.p, w3.p, s);
+        std::vector<u32> perm(static_cast<size_t>(n));
+        uint64_t rng = o->seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        auto next = [&]() {
+            // splitmix64
+            uint64_t z = (rng += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        int step = 0;
+        for (int e = 0; e < o->epochs; e++) {
+            for (uint64_t i = 0; i < n; i++) perm[size_t(i)] = u32(i);
+            for (uint64_t i = n - 1; i > 0; i--) std::swap(perm[size_t(i)], perm[size_t(next() % (i + 1))]);
+            HIP_TRY(hipMemcpyAsync(P.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, s));
+            const float zero = 0.f;
+            HIP_TRY(hipMemcpyAsync(loss.p, &zero, 4, hipMemcpyHostToDevice, s));
+            int batches = 0;
+            for (uint64_t first = 0; first < n; first += uint64_t(o->batch)) {
+                const int rows = int(std::min<uint64_t>(uint64_t(o->batch), n - first));
+                step++;
+                const float bc1 = 1.0f - std::pow(o->beta1, float(step));
+                const float bc2 = 1.0f - std::pow(o->beta2, float(step));
+                launch_mlp_grad(X.p, Y.p, P.p, first, rows, w1.p, w2.p, w3.p, master.p, 1.0f / float(rows), partial.p,
+                                s);
+                launch_mlp_reduce(partial.p, mlp_partials(rows), 1, nullptr, master.p, m1.p, m2.p, o->lr, o->beta1,
+                                  o->beta2, o->eps, bc1, bc2, w1.p, w2.p, w3.p, loss.p, s);
+                batches++;
+            }
+            HIP_TRY(hipGetLastError());
+            float l = 0.f;
+            HIP_TRY(hipMemcpyAsync(&l, loss.p, 4, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (epoch_loss) epoch_loss[e] = l / float(batches);
+        }
+        HIP_TRY(hipMemcpy(params, master.p, sizeof(float) * CARE_MLP_PARAMS, hipMemcpyDeviceToHost));
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_mlp_infer(care_context* ctx, const float* features, uint64_t n, const float* params, float* logits) {
+    if (!ctx || !features || !params || !logits) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (n < 1 || n > (uint64_t(1) << 32)) return fail(CARE_ERR_ARGUMENT, "n out of range");
+    if (!all_finite(features, size_t(n) * CARE_MLP_FEATURES) || !all_finite(params, CARE_MLP_PARAMS))
+        return fail(CARE_ERR_ARGUMENT, "non-finite input");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        const std::vector<unsigned short> xb = to_bf16(features, n);
+        DevBuf<unsigned short> X, w1, w2, w3;
+        DevBuf<float> master, z;
+        X.alloc(xb.size());
+        master.alloc(CARE_MLP_PARAMS);
+        w1.alloc(64 * 32);
+        w2.alloc(32 * 64);
+        w3.alloc(16 * 32);
+        z.alloc(size_t(n));
+        HIP_TRY(hipMemcpy(X.p, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(master.p, params, sizeof(float) * CARE_MLP_PARAMS, hipMemcpyHostToDevice));
+        launch_mlp_cast(master.p, w1.p, w2.p, w3.p, s);
+        launch_mlp_forward(X.p, n, w1.p, w2.p, w3.p, master.p, z.p, s);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(logits, z.p, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_gate_columns(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
+                      uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count) {
+    if (!ctx || !features || !read_ids || !positions || !orig || !cons || !count)
+        return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (ctx->opt.correction_type != 3) return fail(CARE_ERR_STATE, "care_gate_columns needs correction_type 3");
+    if (capacity < 1 || capacity > (uint64_t(1) << 31)) return fail(CARE_ERR_ARGUMENT, "capacity out of range");
+    *count = 0;
+    auto& r = ctx->record;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        if (r.cap < int(capacity)) {
+            r.feat.alloc(size_t(capacity) * CARE_MLP_FEATURES);
+            r.ids.alloc(size_t(capacity));
+            r.pos.alloc(size_t(capacity));
+            r.orig.alloc(size_t(capacity));
+            r.cons.alloc(size_t(capacity));
+            r.cap = int(capacity);
+        }
+        launch_fill_u32(reinterpret_cast<u32*>(ctx->counters.p + 2), 1, 0, ctx->stream);
+        HIP_TRY(hipGetLastError());
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    const int cap = int(std::min<uint64_t>(capacity, uint64_t(r.cap)));
+    const int savedCap = r.cap;
+    r.cap = cap;
+    r.on = true;
+    std::vector<u8> status(size_t(std::max(n, 1)));
+    const int pitch = (ctx->rs.maxLen + 15) / 16 * 16;
+    std::vector<char> seqs(size_t(std::max(n, 1)) * size_t(pitch));
+    care_batch_result res{};
+    res.anchor_status = status.data();
+    res.anchor_sequences = seqs.data();
+    res.sequence_pitch = pitch;
+    const bool cc = ctx->opt.correct_candidates != 0;
+    ctx->opt.correct_candidates = 0;
+    const int rc = care_correct_batch(ctx, first, n, &res);
+    ctx->opt.correct_candidates = cc;
+    r.on = false;
+    r.cap = savedCap;
+    if (rc != CARE_OK) return rc;
+    try {
+        int got = 0;
+        HIP_TRY(hipMemcpy(&got, ctx->counters.p + 2, sizeof(int), hipMemcpyDeviceToHost));
+        if (got > cap) return fail(CARE_ERR_CAPACITY, "%d gate columns exceed capacity %d", got, cap);
+        if (got > 0) {
+            HIP_TRY(hipMemcpy(features, r.feat.p, sizeof(float) * size_t(got) * CARE_MLP_FEATURES, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(read_ids, r.ids.p, sizeof(u32) * size_t(got), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(positions, r.pos.p, sizeof(uint16_t) * size_t(got), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(orig, r.orig.p, size_t(got), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(cons, r.cons.p, size_t(got), hipMemcpyDeviceToHost));
+        }
+        *count = uint64_t(got);
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
     return CARE_OK;
 }
 

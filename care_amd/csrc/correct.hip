@@ -14,6 +14,7 @@
 #include <algorithm>
 
 #include "device_common.h"
+#include "mlp_device.h"
 
 namespace care_engine {
 
@@ -73,10 +74,13 @@ struct Lds {
     u16* cnt;       // [4][cols]
     float* wgt;     // [4][cols]
     int* scratch;   // [32]
+    u16* dec;       // gated mode: decision columns (anchor positions) [maxLen]
+    u8* gacc;       // gated mode: accepted flag per anchor position [maxLen]
+    char* gscr;     // gated mode: feature tile and MLP scratch, overlays mst/rows/qrows
 };
 
 struct LdsDims {
-    int pitch, qpitch, hw, maxLen, cols, waves, chunk;
+    int pitch, qpitch, hw, maxLen, cols, waves, chunk, gated;
     __host__ __device__ int pw() const { return hw + 3; }
     __host__ __device__ size_t bytes() const {
         size_t b = 0;
@@ -90,6 +94,7 @@ struct LdsDims {
         b += sizeof(float) * cols + 2 * sizeof(int) * cols;
         b += 4 * sizeof(u16) * cols + 4 * sizeof(float) * cols;
         b += 32 * sizeof(int);
+        if (gated) b += (sizeof(u16) + 1) * ((size_t(maxLen) + 15) / 16 * 16);
         return b;
     }
 };
@@ -112,7 +117,16 @@ __device__ Lds carve(char* base, const LdsDims& d) {
     l.scratch = reinterpret_cast<int*>(p); p += 32 * sizeof(int);
     l.cnt = reinterpret_cast<u16*>(p); p += 4 * sizeof(u16) * d.cols;
     l.cons = reinterpret_cast<u8*>(p); p += (size_t(d.cols) + 3) / 4 * 4;
-    l.anchorQ = reinterpret_cast<char*>(p);
+    l.anchorQ = reinterpret_cast<char*>(p); p += (size_t(d.maxLen) + 3) / 4 * 4;
+    l.gscr = reinterpret_cast<char*>(l.mst);
+    if (d.gated) {
+        const size_t ml = (size_t(d.maxLen) + 15) / 16 * 16;
+        l.dec = reinterpret_cast<u16*>(p); p += sizeof(u16) * ml;
+        l.gacc = reinterpret_cast<u8*>(p);
+    } else {
+        l.dec = nullptr;
+        l.gacc = nullptr;
+    }
     return l;
 }
 
@@ -195,10 +209,14 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
                     L.qrows[c * qpitch + wQ] = rs.quals[u64(meta[c0 + c].id) * qpitch + wQ];
             __syncthreads();
             if (!active) continue;
+            // branch-free body (base 4 adds nothing), so the loop can be
+            // unrolled and its LDS reads overlapped
+#pragma unroll 4
             for (int c = 0; c < nc; c++) {
                 const CandMeta cm = L.mst[c];
-                const int pos = col - msa.ab - int(cm.shift);
-                if (pos < 0 || pos >= int(cm.len)) continue;
+                int pos = col - msa.ab - int(cm.shift);
+                const bool in = pos >= 0 && pos < int(cm.len);
+                pos = in ? pos : 0;
                 const bool rc = cm.flag == 2;
                 const u32 b = oriented_base(L.rows + c * pitch, cm.len, pos, rc);
                 float w = cm.u.weight;
@@ -207,7 +225,7 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
                     const char q = quality_char(L.qrows + c * qpitch, qi, rs.qbits);
                     w = cm.u.weight * L.qw[(unsigned char)q];
                 }
-                CARE_MSA_ADD(b, w);
+                CARE_MSA_ADD(in ? b : 4u, w);
             }
         }
 #undef CARE_MSA_ADD
@@ -420,8 +438,69 @@ __device__ void column_features(const Lds& L, int cols, int pos, u32 origBase, c
     f[12] = float(cs.minCov) / cov;
 }
 
+// The 32 gate features of anchor position i (column ab + i) whose
+// consensus differs from the anchor base.  `acs` are the anchor-column MSA
+// properties, nd the number of decision columns of the anchor, m the
+// candidates in the MSA.  Shared with the oracle's restatement.
+__device__ void gate_features(const Lds& L, int cols, int ab, int la, int i, int nd, int m, const ColumnStats& acs,
+                              float estCov, bool useQ, float* f) {
+    const int col = ab + i;
+    const u32 o = base_at(L.aw, i);
+    const u32 c = L.cons[col];
+    const float cov = float(L.coverage[col]);
+    const float w0 = L.wgt[0 * cols + col], w1 = L.wgt[1 * cols + col], w2 = L.wgt[2 * cols + col],
+                w3 = L.wgt[3 * cols + col];
+    const float n0 = float(L.cnt[0 * cols + col]), n1 = float(L.cnt[1 * cols + col]), n2 = float(L.cnt[2 * cols + col]),
+                n3 = float(L.cnt[3 * cols + col]);
+    const float wsum = w0 + w1 + w2 + w3;
+    const float wo = o == 0 ? w0 : o == 1 ? w1 : o == 2 ? w2 : w3;
+    const float wc = c == 0 ? w0 : c == 1 ? w1 : c == 2 ? w2 : w3;
+    const float no = o == 0 ? n0 : o == 1 ? n1 : o == 2 ? n2 : n3;
+    const float nc = c == 0 ? n0 : c == 1 ? n1 : c == 2 ? n2 : n3;
+    f[0] = no / cov;
+    f[1] = wo / wsum;
+    f[2] = nc / cov;
+    f[3] = L.support[col];
+    f[4] = wo / fmaxf(no, 1.0f);
+    f[5] = wc / fmaxf(nc, 1.0f);
+    f[6] = cov / estCov;
+    f[7] = wsum / estCov;
+    f[8] = (cov - no - nc) / cov;
+    f[9] = useQ ? L.qw[(unsigned char)L.anchorQ[i]] : 1.0f;
+    f[10] = float(i) / float(la);
+    f[11] = float(min(i, la - 1 - i)) / float(la);
+    f[12] = acs.avgSupport;
+    f[13] = acs.minSupport;
+    f[14] = float(acs.minCov) / estCov;
+    f[15] = float(acs.maxCov) / estCov;
+    f[16] = float(m) / estCov;
+    const int dcol[4] = {-2, -1, 1, 2};
+    for (int k = 0; k < 4; k++) {
+        const int cc = col + dcol[k];
+        f[17 + k] = (cc >= 0 && cc < cols) ? L.support[cc] : 0.0f;
+        const int ii = i + dcol[k];
+        f[21 + k] = (ii >= 0 && ii < la && u32(L.cons[ab + ii]) != base_at(L.aw, ii)) ? 1.0f : 0.0f;
+    }
+    f[25] = float(nd) / float(la);
+    f[26] = col - 1 >= 0 ? float(L.coverage[col - 1]) / estCov : 0.0f;
+    f[27] = col + 1 < cols ? float(L.coverage[col + 1]) / estCov : 0.0f;
+    f[28] = (wc - wo) / wsum;
+    const float q0 = n0 / cov, q1 = n1 / cov, q2 = n2 / cov, q3 = n3 / cov;
+    f[29] = q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3;
+    f[30] = wsum / cov;
+    int near = 0;
+    for (int d = -5; d <= 5; d++) {
+        const int ii = i + d;
+        if (d != 0 && ii >= 0 && ii < la && u32(L.cons[ab + ii]) != base_at(L.aw, ii)) near++;
+    }
+    f[31] = float(near) / 10.0f;
+}
+
 }  // namespace
 
+// kGated: the learned-gate mode (correctionType 3), compiled separately so
+// that its register use does not lower the occupancy of the other modes.
+template <bool kGated>
 __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) {
     HIP_DYNAMIC_SHARED(uint4, ldsRaw)  // uint4: 16-byte aligned dynamic LDS
     const Lds L = carve(reinterpret_cast<char*>(ldsRaw), dims);
@@ -648,7 +727,8 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         CandMeta* cur = metaB;
         CandMeta* other = metaA;
         Msa msa = build_msa(P, L, dims, cur, m, la, &sMin, &sMax);
-        for (int iter = 0; iter < 5 && m > 0; iter++) {
+        const int maxIter = kGated ? 0 : 5;  // the gated mode skips refinement
+        for (int iter = 0; iter < maxIter && m > 0; iter++) {
             // src/msa.cpp:532-734
             if (tid == 0) {
                 sFlag = 0;
@@ -726,7 +806,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         // ---- phase D: anchor correction (src/msa.cpp:304-453, include/corrector.hpp:1423-1472) ----
         // the forest path corrects even when refinement removed every
         // candidate (correctAnchorClf sets isCorrected unconditionally)
-        const bool forestA = P.correctionType == 1;
+        const bool forestA = !kGated && P.correctionType == 1;
         if (m > 0 || forestA) {
             __shared__ float sMinSupport, sAvgSupport;
             __shared__ int sMinCov, sMaxCov, sHQ, sByCons;
@@ -760,6 +840,65 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             acs.minSupport = sMinSupport;
             acs.minCov = sMinCov;
             acs.maxCov = sMaxCov;
+            const bool gated = kGated;
+            if (kGated && !hq) {
+                // decision columns in position order
+                int nd = 0;
+                for (int base = 0; base < la; base += blockDim.x) {
+                    const int i = base + tid;
+                    const bool d = i < la && u32(L.cons[msa.ab + i]) != base_at(L.aw, i);
+                    if (i < la) L.gacc[i] = 0;
+                    int cnt;
+                    const int pos = block_exclusive_count(d, L.scratch, &cnt);
+                    if (d) L.dec[nd + pos] = u16(i);
+                    nd += cnt;
+                }
+                __syncthreads();
+                bf16raw* xt = reinterpret_cast<bf16raw*>(L.gscr);                  // [32][32]
+                float* zt = reinterpret_cast<float*>(L.gscr + 32 * kMlpIn * 2);   // [32]
+                char* tileScr = L.gscr + 32 * kMlpIn * 2 + 32 * 4;                // 2 x kMlpTileScratch
+                float* ft = reinterpret_cast<float*>(tileScr + 2 * kMlpTileScratch);  // [32][33] features
+                for (int g0 = 0; g0 < nd; g0 += 32) {
+                    const int ng = min(32, nd - g0);
+                    if (tid < 32) {
+                        float* f = ft + tid * (kMlpIn + 1);
+                        if (tid < ng) {
+                            gate_features(L, msa.cols, msa.ab, la, int(L.dec[g0 + tid]), nd, m, acs,
+                                          P.estimatedCoverage, P.useQuality != 0, f);
+                        } else {
+                            for (int k = 0; k < kMlpIn; k++) f[k] = 0.0f;
+                        }
+                        for (int k = 0; k < kMlpIn; k++) xt[tid * kMlpIn + k] = f2bf(f[k]);
+                        if (P.gateFeat && tid < ng) {
+                            const int slot = atomicAdd(P.gateCount, 1);
+                            if (slot < P.gateCap) {
+                                const int i = int(L.dec[g0 + tid]);
+                                for (int k = 0; k < kMlpIn; k++) P.gateFeat[u64(slot) * kMlpIn + k] = f[k];
+                                P.gateIds[slot] = u32(id);
+                                P.gatePos[slot] = u16(i);
+                                P.gateOrig[slot] = u8(base_at(L.aw, i));
+                                P.gateCons[slot] = L.cons[msa.ab + i];
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    if (P.gateParams && wave < 2 && 16 * wave < ng) {
+                        MlpWeights W;
+                        W.w1 = P.gateW1;
+                        W.w2 = P.gateW2;
+                        W.w3 = P.gateW3;
+                        W.b1 = P.gateParams + kMlpB1;
+                        W.b2 = P.gateParams + kMlpB2;
+                        W.b3 = P.gateParams + kMlpB3;
+                        W.w4 = P.gateParams + kMlpW4;
+                        W.b4 = P.gateParams[kMlpB4];
+                        mlp_tile_forward(W, xt + 16 * wave * kMlpIn, tileScr + wave * kMlpTileScratch, zt + 16 * wave);
+                    }
+                    __syncthreads();
+                    if (tid < ng) L.gacc[L.dec[g0 + tid]] = (P.gateParams && zt[tid] >= P.gateLogit) ? 1 : 0;
+                    __syncthreads();
+                }
+            }
             for (int i = tid; i < la; i += blockDim.x) {
                 const int col = msa.ab + i;
                 const u32 orig = base_at(L.aw, i);
@@ -771,6 +910,9 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                         column_features(L, msa.cols, col, orig, acs, P.estimatedCoverage, f);
                         if (!forest_decide(P.forestA, f)) b = orig;
                     }
+                } else if (gated) {
+                    b = L.cons[col];
+                    if (!hq && b != orig && !L.gacc[i]) b = orig;
                 } else if (byCons) {
                     b = L.cons[col];
                 } else {
@@ -838,7 +980,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
 
 // LDS shape of k_correct; MSA chunks hold up to 256 candidates and about
 // 16 KiB of staged rows
-static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads) {
+static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads, int gated) {
     LdsDims d;
     d.pitch = (maxLen + 15) / 16;
     d.qpitch = useQuality ? qpitch : 0;
@@ -848,16 +990,25 @@ static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads) {
     d.waves = threads / kWave;
     const int rowBytes = int(sizeof(CandMeta)) + 4 * (d.pitch + d.qpitch);
     d.chunk = std::max(16, std::min(256, 16384 / rowBytes));
+    d.gated = gated;
+    if (gated) {
+        // the staging area doubles as the gate's feature tile and MLP scratch
+        const int need = 32 * kMlpIn * 2 + 32 * 4 + 2 * kMlpTileScratch + 32 * (kMlpIn + 1) * 4;
+        while (d.chunk * rowBytes < need) d.chunk++;
+    }
     return d;
 }
 
-size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads) {
-    return make_dims(maxLen, useQuality, qpitch, threads).bytes();
+size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads, int gated) {
+    return make_dims(maxLen, useQuality, qpitch, threads, gated).bytes();
 }
 
 void launch_correct(const CorrectParams& p, int blocks, int threads, hipStream_t stream) {
-    const LdsDims d = make_dims(p.rs.maxLen, p.useQuality, p.rs.qpitch, threads);
-    hipLaunchKernelGGL(k_correct, dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
+    const LdsDims d = make_dims(p.rs.maxLen, p.useQuality, p.rs.qpitch, threads, p.correctionType == 3);
+    if (p.correctionType == 3)
+        hipLaunchKernelGGL(k_correct<true>, dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
+    else
+        hipLaunchKernelGGL(k_correct<false>, dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
 }
 
 }  // namespace care_engine

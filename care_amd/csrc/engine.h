@@ -128,6 +128,22 @@ struct CorrectParams {
     DevForest forestA;
     DevForest forestC;
     float estimatedCoverage;
+    // learned gate (correctionType 3): BF16 weight matrices and the FP32
+    // parameter vector of the gate, logit threshold of acceptance
+    const unsigned short* gateW1;
+    const unsigned short* gateW2;
+    const unsigned short* gateW3;
+    const float* gateParams;
+    float gateLogit;
+    // optional record of every gate decision column (features, read id,
+    // position, original and consensus base); null when not recording
+    float* gateFeat;  // [gateCap][32]
+    u32* gateIds;
+    unsigned short* gatePos;
+    u8* gateOrig;
+    u8* gateCons;
+    int* gateCount;
+    int gateCap;
 };
 
 // launchers (all asynchronous on `stream`)
@@ -135,9 +151,23 @@ void launch_signatures(const DevReadStore& rs, u64 first, int n, int k, int h, u
 void launch_build_map(const DevReadStore& rs, const u64* sigs, int h, int map, int cap, u64* tkeys, u32* tcount,
                       u64 tmask, u32* tfill, u64* scanTmp, u32* scanOut, hipStream_t stream);
 void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, int maxCand, int excludeAmbiguous,
-                  u32* candIds, int* candCount, unsigned long long* rawHits, hipStream_t stream);
+                  int candLimit, u32* candIds, int* candCount, unsigned long long* rawHits, hipStream_t stream);
 void launch_correct(const CorrectParams& p, int blocks, int threads, hipStream_t stream);
-size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads);
-int query_lds_bytes(int h, int cap, int maxLen, int threads);
+size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads, int gated);
+int query_lds_bytes(int h, int cap, int maxLen, int threads, int candLimit);
+
+// learned gate (mlp.hip)
+void launch_mlp_grad(const unsigned short* X, const u8* Y, const u32* perm, u64 first, int rows,
+                     const unsigned short* w1b, const unsigned short* w2b, const unsigned short* w3b,
+                     const float* master, float invBatch, float* partial, hipStream_t s);
+int mlp_partials(int rows);
+int mlp_partial_stride();
+void launch_mlp_reduce(const float* partial, int nPartials, int mode, float* out, float* master, float* m1, float* m2,
+                       float lr, float beta1, float beta2, float eps, float bc1, float bc2, unsigned short* w1b,
+                       unsigned short* w2b, unsigned short* w3b, float* lossAcc, hipStream_t s);
+void launch_mlp_cast(const float* master, unsigned short* w1b, unsigned short* w2b, unsigned short* w3b,
+                     hipStream_t s);
+void launch_mlp_forward(const unsigned short* X, u64 n, const unsigned short* w1b, const unsigned short* w2b,
+                        const unsigned short* w3b, const float* master, float* zOut, hipStream_t s);
 
 }  // namespace care_engine

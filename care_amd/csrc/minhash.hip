@@ -264,8 +264,10 @@ void launch_sort_buckets(const u32* toff, const u32* tcount, u64 tsize, u32* val
 // Query: one block per anchor.  Signature, probes of all maps, gather of
 // the hit lists into LDS, bitonic sort, unique, self/ambiguous removal.
 // ---------------------------------------------------------------------------
+// With candLimit > 0 only the candLimit candidates hit by the most maps are
+// kept (ties broken by smaller read id), still in ascending id order.
 __global__ void k_query(DevReadStore rs, DevTables t, u64 first, int n, int maxCand, int sortCap, int excludeAmbiguous,
-                        u32* candIds, int* candCount, unsigned long long* rawHits) {
+                        int candLimit, u32* candIds, int* candCount, unsigned long long* rawHits) {
     HIP_DYNAMIC_SHARED(u64, lds64)
     u64* canon = lds64;                                  // maxLen
     u64* partial = canon + rs.maxLen;                    // blockDim
@@ -273,7 +275,9 @@ __global__ void k_query(DevReadStore rs, DevTables t, u64 first, int n, int maxC
     int* counts = reinterpret_cast<int*>(sig + t.h);     // h + 1 (prefix)
     int* scanScratch = counts + t.h + 2;                 // 17
     u32* ids = reinterpret_cast<u32*>(scanScratch + 18); // sortCap
-    __shared__ int sTotal;
+    u8* hits = reinterpret_cast<u8*>(ids + sortCap);      // maxCand (candLimit > 0 only)
+    int* hist = counts;                                  // h + 1 bins, reused after the gather
+    __shared__ int sTotal, sThresh, sNeed;
 
     for (int a = blockIdx.x; a < n; a += gridDim.x) {
         const u64 id = first + u64(a);
@@ -360,31 +364,75 @@ __global__ void k_query(DevReadStore rs, DevTables t, u64 first, int n, int maxC
             }
             int cnt;
             const int pos = block_exclusive_count(keep, scanScratch, &cnt);
-            if (keep && written + pos < maxCand) out[written + pos] = v;
+            if (keep && written + pos < maxCand) {
+                out[written + pos] = v;
+                if (candLimit > 0) {
+                    int run = 1;  // maps that hit v: length of its run in the sorted hits
+                    while (i + run < total && ids[i + run] == v) run++;
+                    hits[written + pos] = u8(run);
+                }
+            }
             written += cnt;
         }
+        if (written > maxCand) written = maxCand;
+        if (candLimit > 0 && written > candLimit) {
+            // threshold hit count T: keep every candidate above T and the
+            // first (in id order) candidates at T up to candLimit
+            __syncthreads();
+            for (int b = threadIdx.x; b <= t.h; b += blockDim.x) hist[b] = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < written; i += blockDim.x) atomicAdd(&hist[hits[i]], 1);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int above = 0, v = t.h;
+                while (v > 0 && above + hist[v] < candLimit) above += hist[v--];
+                sThresh = v;
+                sNeed = candLimit - above;
+            }
+            __syncthreads();
+            const int T = sThresh, need = sNeed;
+            int kept = 0, atT = 0;
+            for (int base = 0; base < written; base += blockDim.x) {
+                const int i = base + threadIdx.x;
+                u32 v = 0;
+                int hc = 0;
+                if (i < written) {
+                    v = out[i];
+                    hc = hits[i];
+                }
+                int nT;
+                const int rT = block_exclusive_count(i < written && hc == T, scanScratch, &nT);
+                const bool keep = i < written && (hc > T || (hc == T && atT + rT < need));
+                int cnt;
+                const int pos = block_exclusive_count(keep, scanScratch, &cnt);
+                if (keep) out[kept + pos] = v;
+                kept += cnt;
+                atT += nT;
+            }
+            written = kept;
+        }
         if (threadIdx.x == 0) {
-            const int c = written < maxCand ? written : maxCand;
-            candCount[a] = c;
-            if (rawHits) atomicAdd(rawHits + 1, (unsigned long long)c);
+            candCount[a] = written;
+            if (rawHits) atomicAdd(rawHits + 1, (unsigned long long)written);
         }
         __syncthreads();
     }
 }
 
-int query_lds_bytes(int h, int cap, int maxLen, int threads) {
+int query_lds_bytes(int h, int cap, int maxLen, int threads, int candLimit) {
     const int sortCap = int(1u << (32 - __builtin_clz(unsigned(h * cap > 1 ? h * cap - 1 : 1))));
-    return int(sizeof(u64) * (size_t(maxLen) + threads + h) + sizeof(int) * (h + 2 + 18) + sizeof(u32) * sortCap);
+    return int(sizeof(u64) * (size_t(maxLen) + threads + h) + sizeof(int) * (h + 2 + 18) + sizeof(u32) * sortCap +
+               (candLimit > 0 ? size_t(h) * cap : 0));
 }
 
 void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, int maxCand, int excludeAmbiguous,
-                  u32* candIds, int* candCount, unsigned long long* rawHits, hipStream_t stream) {
+                  int candLimit, u32* candIds, int* candCount, unsigned long long* rawHits, hipStream_t stream) {
     if (n <= 0) return;
     const int threads = 256;
-    const int lds = query_lds_bytes(t.h, maxCand / t.h, rs.maxLen, threads);
+    const int lds = query_lds_bytes(t.h, maxCand / t.h, rs.maxLen, threads, candLimit);
     const int sortCap = int(1u << (32 - __builtin_clz(unsigned(maxCand > 1 ? maxCand - 1 : 1))));
     hipLaunchKernelGGL(k_query, dim3(n), dim3(threads), lds, stream, rs, t, first, n, maxCand, sortCap,
-                       excludeAmbiguous, candIds, candCount, rawHits);
+                       excludeAmbiguous, candLimit, candIds, candCount, rawHits);
 }
 
 }  // namespace care_engine

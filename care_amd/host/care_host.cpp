@@ -309,6 +309,7 @@ struct HostOptions {
     std::vector<int> gpus;
     int batchsize = 0;
     std::string forestAnchor, forestCands;
+    std::string gateFile;  // learned gate parameters (correctionType 3)
     uint32_t maxTreesAnchor = 0xFFFFFFFFu, maxTreesCands = 0xFFFFFFFFu;
     bool haveCoverage = false, havePairmode = false;
 };
@@ -375,6 +376,8 @@ HostOptions parseOptions(int argc, char** argv) {
         else if (a == "--correctionType") o.eng.correction_type = std::stoi(value());
         else if (a == "--correctionTypeCands") o.eng.correction_type_cands = std::stoi(value());
         else if (a == "--ml-forestfile") o.forestAnchor = value();
+        else if (a == "--ml-gatefile") o.gateFile = value();
+        else if (a == "--candidateLimit") o.eng.candidate_limit = std::stoi(value());
         else if (a == "--ml-cands-forestfile") o.forestCands = value();
         else if (a == "--thresholdAnchor") o.eng.threshold_anchor = std::stof(value());
         else if (a == "--thresholdCands") o.eng.threshold_cands = std::stof(value());
@@ -564,6 +567,18 @@ int runMain(int argc, char** argv) {
     std::unique_ptr<care_context, int (*)(care_context*)> guard(ctx, care_context_destroy);
     if (o.batchsize > 0) check(care_set_launch_config(ctx, (o.batchsize + 1) & ~1, 0));
     if (o.eng.correction_type == 1) check(care_forest_load(ctx, 0, o.forestAnchor.c_str(), o.maxTreesAnchor));
+    if (o.eng.correction_type == 3) {
+        // gate parameters: CARE_MLP_PARAMS little-endian float32 values
+        std::vector<float> params(CARE_MLP_PARAMS);
+        FILE* f = std::fopen(o.gateFile.c_str(), "rb");
+        if (!f) throw std::runtime_error("cannot open gate file '" + o.gateFile + "' (--ml-gatefile)");
+        const size_t got = std::fread(params.data(), sizeof(float), params.size(), f);
+        const bool extra = std::fgetc(f) != EOF;
+        std::fclose(f);
+        if (got != params.size() || extra)
+            throw std::runtime_error("gate file '" + o.gateFile + "' must hold exactly CARE_MLP_PARAMS float32 values");
+        check(care_mlp_load(ctx, params.data()));
+    }
     if (o.eng.correct_candidates && o.eng.correction_type_cands == 1)
         check(care_forest_load(ctx, 1, o.forestCands.c_str(), o.maxTreesCands));
     check(care_readstore_upload(ctx, rs->n, rs->maxLen, rs->seqs.data(), rs->pitch, rs->lens.data(), rs->ambig.data(),

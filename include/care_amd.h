@@ -45,10 +45,12 @@ typedef struct care_options {
     float error_factor;          /* --errorfactortuning (estimatedErrorrate) */
     float coverage_factor;       /* --coveragefactortuning (m_coverage) */
     float paired_filter_threshold; /* --pairedFilterThreshold */
-    int correction_type;         /* --correctionType: 0 classic, 1 forest, 3 mlp */
-    int correction_type_cands;   /* --correctionTypeCands: 0 classic, 1 forest, 3 mlp */
+    int correction_type;         /* --correctionType: 0 classic, 1 forest, 3 learned gate */
+    int correction_type_cands;   /* --correctionTypeCands: 0 classic, 1 forest */
     float threshold_anchor;      /* --thresholdAnchor */
     float threshold_cands;       /* --thresholdCands */
+    int candidate_limit;         /* correction_type 3 only: keep the candidates hit by the
+                                    most maps, at most this many per anchor (0 = all) */
 } care_options;
 
 typedef struct care_context care_context;
@@ -131,6 +133,53 @@ int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threa
  * which: 0 anchor, 1 candidates.  Replaces GpuForest
  * (include/gpu/forest_gpu.cuh:133). */
 int care_forest_load(care_context* ctx, int which, const char* path, uint32_t max_trees);
+
+/* ---- Learned correction gate (correction_type 3) ----------------------
+ * No counterpart in the reference: a 32-64-32-16-1 MLP (ReLU, sigmoid
+ * output) over MSA column features replaces the forest decision of
+ * correctAnchorClf (include/corrector.hpp:1448-1472).  In correction_type 3
+ * the engine also keeps only the candidate_limit candidates hit by the most
+ * maps and skips MSA refinement; every anchor column whose consensus
+ * differs from the anchor base in a non-HQ MSA is corrected only if the
+ * gate's output reaches threshold_anchor.  Parameters are one FP32 vector of
+ * CARE_MLP_PARAMS values: W1[64][32], b1[64], W2[32][64], b2[32],
+ * W3[16][32], b3[16], w4[16], b4 (matrices output-major).  The device
+ * evaluates it with BF16 inputs and weights and FP32 accumulation. */
+#define CARE_MLP_FEATURES 32
+#define CARE_MLP_PARAMS 4737
+
+typedef struct care_mlp_train_options {
+    int epochs;      /* passes over the data */
+    int batch;       /* rows per Adam step, 16..65536 */
+    float lr, beta1, beta2, eps;  /* Adam */
+    uint64_t seed;   /* shuffling */
+} care_mlp_train_options;
+
+/* Sets the gate used by care_correct_batch under correction_type 3. */
+int care_mlp_load(care_context* ctx, const float* params);
+
+/* Gradient of the mean binary cross-entropy over n rows (features [n][32],
+ * labels 0/1) at params, computed by the device's training kernels; loss_out
+ * may be NULL. */
+int care_mlp_gradient(care_context* ctx, const float* features, const uint8_t* labels, int n, const float* params,
+                      float* grad_out, float* loss_out);
+
+/* Trains on the device with Adam from the parameters in params (overwritten
+ * with the result); epoch_loss (may be NULL) receives the mean batch loss of
+ * every epoch. */
+int care_mlp_train(care_context* ctx, const float* features, const uint8_t* labels, uint64_t n,
+                   const care_mlp_train_options* opt, float* params, float* epoch_loss);
+
+/* Gate outputs before the sigmoid (logits) for n feature rows. */
+int care_mlp_infer(care_context* ctx, const float* features, uint64_t n, const float* params, float* logits);
+
+/* Runs the correction_type 3 pipeline on anchors [first, first+n) and
+ * returns every gate decision column: its 32 features, anchor read id,
+ * position, original and consensus base (0..3).  Works without loaded gate
+ * weights (then no column is corrected).  *count receives the number of
+ * columns; CARE_ERR_CAPACITY if they exceed capacity. */
+int care_gate_columns(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
+                      uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count);
 
 /* Timing and work counters of the last care_correct_batch call. */
 typedef struct care_batch_stats {

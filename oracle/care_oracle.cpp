@@ -61,7 +61,10 @@ struct Options {
     int new_columns_to_correct = 15;
     int kmerlength = 20;
     int numHashFunctions = 48;
-    int correctionType = 0;      // 0 classic, 1 forest
+    int correctionType = 0;      // 0 classic, 1 forest, 3 learned gate (engine extension)
+    int candidateLimit = 0;      // correctionType 3: top candidates by map hits
+    int gateConstant = 1;        // correctionType 3: the oracle's gate accepts (1) or rejects (-1) every column
+    std::string gateDump;        // correctionType 3: binary record of every decision column
     int correctionTypeCands = 0;
     float thresholdAnchor = 0.5f;
     float thresholdCands = 0.5f;
@@ -941,6 +944,25 @@ struct Corrector {
         if (opt.excludeAmbiguousReads && rs.ambig[id]) return ids;
         tables.query(ids, rs.seq(id), rs.lens[id]);
         std::sort(ids.begin(), ids.end());
+        if (opt.correctionType == 3 && opt.candidateLimit > 0) {
+            // the engine's gated mode: keep the candidates hit by the most
+            // maps (ties to the smaller id), after self / ambiguous removal
+            std::vector<std::pair<int, u32>> hc;
+            for (std::size_t i = 0; i < ids.size();) {
+                std::size_t j = i;
+                while (j < ids.size() && ids[j] == ids[i]) j++;
+                if (ids[i] != id && !(opt.excludeAmbiguousReads && rs.ambig[ids[i]])) hc.push_back({int(j - i), ids[i]});
+                i = j;
+            }
+            if (int(hc.size()) > opt.candidateLimit) {
+                std::stable_sort(hc.begin(), hc.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+                hc.resize(std::size_t(opt.candidateLimit));
+                std::sort(hc.begin(), hc.end(), [](const auto& a, const auto& b) { return a.second < b.second; });
+            }
+            ids.clear();
+            for (const auto& e : hc) ids.push_back(e.second);
+            return ids;
+        }
         ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
         auto it = std::lower_bound(ids.begin(), ids.end(), id);
         if (it != ids.end() && *it == id) ids.erase(it);
@@ -961,12 +983,76 @@ struct Corrector {
         std::string qual;
     };
 
+    struct GateColumn {
+        u32 id;
+        std::uint16_t pos;
+        std::uint8_t orig, cons;
+        float f[32];
+    };
+
     struct Output {
         bool hasAnchor = false;
         Result anchor;
         std::vector<Result> cands;
         bool hq = false;
+        std::vector<GateColumn> gate;
     };
+
+    static int baseCode(char c) { return c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : 0; }
+
+    // The engine's 32 gate features of anchor position i (no counterpart in
+    // the reference; restated from care_amd/csrc/correct.hip gate_features).
+    void gateFeatures(float* f, const MSA& msa, const MSA::Props& p, const std::string& anchor, const std::string& anchorQ,
+                      int i, int nd, int m) const {
+        const int col = msa.anchorBegin + i;
+        const int cols = msa.nColumns;
+        const int la = int(anchor.size());
+        const float estCov = opt.estimatedCoverage;
+        const char o = anchor[i], c = msa.consensus[col];
+        const float cov = float(msa.coverage[col]);
+        const float wsum = msa.wA[col] + msa.wC[col] + msa.wG[col] + msa.wT[col];
+        const float wo = weightOf(msa, o, col), wc = weightOf(msa, c, col);
+        const float no = countOf(msa, o, col), nc = countOf(msa, c, col);
+        auto mism = [&](int ii) { return msa.consensus[msa.anchorBegin + ii] != anchor[ii]; };
+        f[0] = no / cov;
+        f[1] = wo / wsum;
+        f[2] = nc / cov;
+        f[3] = msa.support[col];
+        f[4] = wo / std::max(no, 1.0f);
+        f[5] = wc / std::max(nc, 1.0f);
+        f[6] = cov / estCov;
+        f[7] = wsum / estCov;
+        f[8] = (cov - no - nc) / cov;
+        f[9] = opt.useQualityScores ? qweights()(anchorQ[i]) : 1.0f;
+        f[10] = float(i) / float(la);
+        f[11] = float(std::min(i, la - 1 - i)) / float(la);
+        f[12] = p.avg_support;
+        f[13] = p.min_support;
+        f[14] = float(p.min_coverage) / estCov;
+        f[15] = float(p.max_coverage) / estCov;
+        f[16] = float(m) / estCov;
+        const int dcol[4] = {-2, -1, 1, 2};
+        for (int k = 0; k < 4; k++) {
+            const int cc = col + dcol[k];
+            f[17 + k] = (cc >= 0 && cc < cols) ? msa.support[cc] : 0.0f;
+            const int ii = i + dcol[k];
+            f[21 + k] = (ii >= 0 && ii < la && mism(ii)) ? 1.0f : 0.0f;
+        }
+        f[25] = float(nd) / float(la);
+        f[26] = col - 1 >= 0 ? float(msa.coverage[col - 1]) / estCov : 0.0f;
+        f[27] = col + 1 < cols ? float(msa.coverage[col + 1]) / estCov : 0.0f;
+        f[28] = (wc - wo) / wsum;
+        const float q0 = float(msa.cA[col]) / cov, q1 = float(msa.cC[col]) / cov, q2 = float(msa.cG[col]) / cov,
+                    q3 = float(msa.cT[col]) / cov;
+        f[29] = q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3;
+        f[30] = wsum / cov;
+        int near = 0;
+        for (int d = -5; d <= 5; d++) {
+            const int ii = i + d;
+            if (d != 0 && ii >= 0 && ii < la && mism(ii)) near++;
+        }
+        f[31] = float(near) / 10.0f;
+    }
 
     // include/corrector.hpp:145-323 (process) and :325-536 (processMulti),
     // for one anchor whose candidate list and pair flags are given
@@ -1089,8 +1175,8 @@ struct Corrector {
         };
         buildMsa();
 
-        // refinement (include/corrector.hpp:1307-1421)
-        for (int it = 0; it < 5; it++) {
+        // refinement (include/corrector.hpp:1307-1421); none in the gated mode
+        for (int it = 0; it < (opt.correctionType == 3 ? 0 : 5); it++) {
             std::vector<char> remove;
             if (!msa.differentRegion(int(opt.estimatedCoverage), remove)) break;
             std::vector<Cand> kept;
@@ -1115,6 +1201,26 @@ struct Corrector {
                         anchorFeatures(f, msa, props, i);
                         if (!forestA->decide(f)) corrected[i] = anchor[i];
                     }
+                }
+            }
+            isCorrected = true;
+        } else if (opt.correctionType == 3) {
+            // the engine's gated mode: consensus, and in a non-HQ MSA every
+            // column that differs from the anchor is left to the gate
+            corrected.assign(msa.consensus.begin() + msa.anchorBegin, msa.consensus.begin() + msa.anchorBegin + la);
+            if (!props.isHQ) {
+                std::vector<int> dec;
+                for (int i = 0; i < la; i++)
+                    if (anchor[i] != msa.consensus[msa.anchorBegin + i]) dec.push_back(i);
+                for (int i : dec) {
+                    GateColumn g;
+                    g.id = id;
+                    g.pos = std::uint16_t(i);
+                    g.orig = std::uint8_t(baseCode(anchor[i]));
+                    g.cons = std::uint8_t(baseCode(msa.consensus[msa.anchorBegin + i]));
+                    gateFeatures(g.f, msa, props, anchor, anchorQ, i, int(dec.size()), int(cands.size()));
+                    out.gate.push_back(g);
+                    if (opt.gateConstant < 0) corrected[i] = anchor[i];
                 }
             }
             isCorrected = true;
@@ -1400,6 +1506,9 @@ Options parseArgs(int argc, char** argv) {
         else if (a == "--correctionQualityLabels") o.outputCorrectionQualityLabels = boolFlag();
         else if (a == "--correctionType") o.correctionType = std::stoi(next());
         else if (a == "--correctionTypeCands") o.correctionTypeCands = std::stoi(next());
+        else if (a == "--candidateLimit") o.candidateLimit = std::stoi(next());
+        else if (a == "--gateConstant") o.gateConstant = std::stoi(next());
+        else if (a == "--gateDump") o.gateDump = next();
         else if (a == "--ml-forestfile") o.forestAnchor = next();
         else if (a == "--ml-cands-forestfile") o.forestCands = next();
         else if (a == "--thresholdAnchor") o.thresholdAnchor = std::stof(next());
@@ -1470,6 +1579,19 @@ int runMain(int argc, char** argv) {
     std::vector<AnchorDump> dumps;
     corr.run(numAnchors, outs, opt.dumpFile.empty() ? nullptr : &dumps);
     const double t2 = now();
+    if (!opt.gateDump.empty()) {
+        // per decision column: u32 read id, u16 position, u8 original and
+        // consensus base, 32 float features; anchors in order
+        std::ofstream g(opt.gateDump, std::ios::binary);
+        for (std::size_t a = 0; a < numAnchors; a++)
+            for (const auto& c : outs[a].gate) {
+                g.write(reinterpret_cast<const char*>(&c.id), 4);
+                g.write(reinterpret_cast<const char*>(&c.pos), 2);
+                g.write(reinterpret_cast<const char*>(&c.orig), 1);
+                g.write(reinterpret_cast<const char*>(&c.cons), 1);
+                g.write(reinterpret_cast<const char*>(c.f), sizeof(c.f));
+            }
+    }
 
     // results per read; HQ anchors with zero edits are not stored and
     // candidate results of reads corrected as HQ anchors are dropped
@@ -1614,6 +1736,31 @@ int orc_tables_query(void* t, const std::uint32_t* enc, int len, std::uint32_t s
     ids.erase(std::remove(ids.begin(), ids.end(), self), ids.end());
     const int n = int(ids.size());
     for (int i = 0; i < n && i < cap; i++) out[i] = ids[i];
+    return n;
+}
+// Candidate limit of the engine's gated fast path (no counterpart in the
+// reference): of the de-duplicated candidates keep the `limit` ones hit by
+// the most maps, ties going to the smaller read id, returned in ascending id
+// order.
+int orc_tables_query_limited(void* t, const std::uint32_t* enc, int len, std::uint32_t self, int limit,
+                             std::uint32_t* out, int cap) {
+    std::vector<std::uint32_t> raw;
+    static_cast<orc::Tables*>(t)->query(raw, enc, len);
+    std::sort(raw.begin(), raw.end());
+    std::vector<std::pair<int, std::uint32_t>> hc;  // (hits, id)
+    for (std::size_t i = 0; i < raw.size();) {
+        std::size_t j = i;
+        while (j < raw.size() && raw[j] == raw[i]) j++;
+        if (raw[i] != self) hc.push_back({int(j - i), raw[i]});
+        i = j;
+    }
+    if (limit > 0 && int(hc.size()) > limit) {
+        std::stable_sort(hc.begin(), hc.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        hc.resize(std::size_t(limit));
+        std::sort(hc.begin(), hc.end(), [](const auto& a, const auto& b) { return a.second < b.second; });
+    }
+    const int n = int(hc.size());
+    for (int i = 0; i < n && i < cap; i++) out[i] = hc[std::size_t(i)].second;
     return n;
 }
 int orc_tables_bucket(void* t, int map, std::uint64_t key, std::uint32_t* out, int cap) {

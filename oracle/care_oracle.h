@@ -26,6 +26,7 @@ void* orc_tables_build(const uint32_t* seqs, int pitch, const int* lens, size_t 
 void orc_tables_free(void* t);
 int orc_tables_query(void* t, const uint32_t* enc, int len, uint32_t self, uint32_t* out, int cap);
 int orc_tables_bucket(void* t, int map, uint64_t key, uint32_t* out, int cap);
+int orc_tables_query_limited(void* t, const uint32_t* enc, int len, uint32_t self, int limit, uint32_t* out, int cap);
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,7 @@ struct Fiber {
 // lane has read it, so one barrier per operation suffices.
 struct Wave {
     unsigned long long vals[2][kWaveSize];
+    unsigned short mf[2][kWaveSize][16];  // matrix operands of the lanes
     int arrived = 0;
     int size = 0;
     unsigned long gen = 0;
@@ -168,6 +169,39 @@ inline T exchange(T v, int src) {
     w.vals[g & 1][lane()] = to_bits(v);
     wave_barrier();
     return from_bits<T>(w.vals[g & 1][src]);
+}
+
+// 16x16 output tile of a BF16 matrix instruction with K = 4 * per: lane l
+// holds A[l%16][per*(l/16) ..], B[per*(l/16) ..][l%16] and
+// D[4(l/16) .. +3][l%16]; products summed in FP32 in k order.
+typedef float emu_f4 __attribute__((vector_size(16)));
+inline emu_f4 mfma_bf16(const unsigned short* a, const unsigned short* b, int per, emu_f4 c) {
+    Wave& w = my_wave();
+    const unsigned long g = w.gen;
+    const int l = lane();
+    for (int j = 0; j < per; j++) {
+        w.mf[g & 1][l][j] = a[j];
+        w.mf[g & 1][l][8 + j] = b[j];
+    }
+    wave_barrier();
+    auto f = [](unsigned short v) {
+        const unsigned u = unsigned(v) << 16;
+        float x;
+        std::memcpy(&x, &u, 4);
+        return x;
+    };
+    emu_f4 d;
+    for (int i = 0; i < 4; i++) {
+        const int row = 4 * (l / 16) + i, col = l % 16;
+        float acc = 0.f;
+        for (int k = 0; k < 4 * per; k++) {
+            const float av = f(w.mf[g & 1][row + 16 * (k / per)][k % per]);
+            const float bv = f(w.mf[g & 1][col + 16 * (k / per)][8 + k % per]);
+            acc += av * bv;
+        }
+        d[i] = c[i] + acc;
+    }
+    return d;
 }
 
 inline void fiber_entry() {
@@ -329,6 +363,25 @@ inline unsigned long long __ballot(int pred) {
     return m;
 }
 inline int __any(int pred) { return __ballot(pred) != 0ull; }
+#define __bf16 unsigned short
+typedef unsigned short emu_us8 __attribute__((vector_size(16)));
+typedef short emu_s4 __attribute__((vector_size(8)));
+inline emu::emu_f4 __builtin_amdgcn_mfma_f32_16x16x32_bf16(emu_us8 a, emu_us8 b, emu::emu_f4 c, int, int, int) {
+    unsigned short av[8], bv[8];
+    for (int j = 0; j < 8; j++) {
+        av[j] = a[j];
+        bv[j] = b[j];
+    }
+    return emu::mfma_bf16(av, bv, 8, c);
+}
+inline emu::emu_f4 __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(emu_s4 a, emu_s4 b, emu::emu_f4 c, int, int, int) {
+    unsigned short av[4], bv[4];
+    for (int j = 0; j < 4; j++) {
+        av[j] = (unsigned short)a[j];
+        bv[j] = (unsigned short)b[j];
+    }
+    return emu::mfma_bf16(av, bv, 4, c);
+}
 inline int __all(int pred) {
     emu::Wave& w = emu::my_wave();
     const unsigned long long full = w.size >= 64 ? ~0ull : ((1ull << w.size) - 1);

@@ -61,6 +61,8 @@ def oracle():
     lib.orc_tables_query.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p,
                                      ctypes.c_int]
     lib.orc_tables_bucket.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int]
+    lib.orc_tables_query_limited.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
+                                             ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
     return lib
 
 
