@@ -9,7 +9,13 @@ read store and the minhash tables are resident in HBM before timing.
 
 One step = correcting every read of the set once (the reference's STEP2):
 minhash query, alignment, filters, MSA, refinement and anchor correction on
-the device, results copied back to host memory.
+the device, results copied back to host memory.  Two paths are timed on the
+same resident read set: the plain path (bit-identical to care-cpu) and the
+MLP-gated path (top candidates by map hits, no refinement, a learned gate
+on every non-HQ decision column; weights care_amd/gate_repeat_pe.bin,
+trained on other seeds of the same generator).  `value` is the gated
+path's reads/s; both paths' reads/s and their base-level TP/FP/FN on the
+read set's truth (care-gpu command line output) are in `config`.
 
 Prints one JSON line (see the fields below).  --gpus above 1 is refused: this
 build runs on one device.
@@ -32,6 +38,7 @@ WORKLOADS = {
         "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "2", "-k", "20", "-h", "32"],
         "options": dict(coverage=30.0, paired_end=1, use_quality=1, kmer_length=20, num_hash_maps=32),
         "qual_bits": 2,
+        "max_length": 150,
         "label": "repeat-pe: 5 Mbp genome, 10 repeat families, 2x150 bp PE at 30x, -q 2-bit, k=20, h=32",
     },
     "uniform-se": {
@@ -39,6 +46,7 @@ WORKLOADS = {
         "args": ["-c", "30", "--pairmode", "SE", "-k", "20", "-h", "16"],
         "options": dict(coverage=30.0, paired_end=0, use_quality=0, kmer_length=20, num_hash_maps=16),
         "qual_bits": 0,
+        "max_length": 100,
         "label": "uniform-se: 1 Mbp genome, 100 bp SE at 30x (300k reads), k=20, h=16",
     },
 }
@@ -92,6 +100,49 @@ def cpu_baseline(reads, w, sample, threads):
                       "query+correction %.2f s on %d threads, %s" % (sample, float(m.group(1)), threads, model)}
 
 
+def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0):
+    """Average wall time of one correction step over all reads (reads and
+    tables resident in HBM), the correction kernel's HIP-event time and
+    launch count, and the candidates of the last step."""
+    eng = care_amd.Engine(opt, device=0)
+    if batch:
+        eng.set_launch_config(batch, 0)
+    eng.upload_readset(rs)
+    eng.build()
+    if gate is not None:
+        eng.mlp_load(gate)
+    n = rs.n
+    for i in range(warmup):
+        eng.correct(0, n)
+    step_ms, corr_ms, query_ms = [], [], []
+    st = None
+    status = None
+    for i in range(steps):
+        ts = time.perf_counter()
+        status, _, _, _ = eng.correct(0, n)
+        te = time.perf_counter()
+        st = eng.stats()
+        step_ms.append((te - ts) * 1e3)
+        corr_ms.append(st.correct_ms)
+        query_ms.append(st.query_ms)
+        log("bench: %s step %d %.1f ms (query %.1f ms, correct kernel %.1f ms)" % (
+            "gated" if gate is not None else "plain", i + 1, step_ms[-1], st.query_ms, st.correct_ms))
+    eng.close()
+    return {"ms": sum(step_ms) / len(step_ms), "correct_ms": sum(corr_ms) / len(corr_ms),
+            "query_ms": sum(query_ms) / len(query_ms), "launches": max(1, st.launches), "candidates": int(st.candidates),
+            "corrected": int((status & 1).sum())}
+
+
+def quality(care_amd, reads, truth, w, extra, outdir):
+    """Base-level TP/FP/FN of the care-gpu command line's output."""
+    cmd = [care_amd.CARE_GPU, "-i", reads, "-o", "c.fastq", "-d", outdir, "-g", "0", "-t", "16"] + w["args"] + extra
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(p.stdout[-2000:])
+    r = care_amd.evaluate(reads, os.path.join(outdir, "c.fastq"), truth)
+    return {k: r[k] for k in ("TP", "FP", "FN")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +153,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--batch", type=int, default=0)
     args = ap.parse_args()
 
@@ -112,83 +164,93 @@ def main():
         sys.exit(2)
 
     import care_amd
+    from care_amd import gate as gate_mod
     w = WORKLOADS[args.workload]
     reads, truth = prepare(args.workload, args.cache)
     t0 = time.perf_counter()
     rs = care_amd.ReadSet([reads], paired_end=bool(w["options"]["paired_end"]), qual_bits=w["qual_bits"])
-    opt = care_amd.default_options(**w["options"])
-    eng = care_amd.Engine(opt, device=0)
-    if args.batch:
-        eng.set_launch_config(args.batch, 0)
-    eng.upload_readset(rs)
-    eng.build()
-    t1 = time.perf_counter()
     n = rs.n
-    log("bench: %d reads packed, uploaded and indexed in %.2f s" % (n, t1 - t0))
+    log("bench: %d reads packed in %.2f s" % (n, time.perf_counter() - t0))
 
-    for i in range(args.warmup):
-        eng.correct(0, n)
-        log("bench: warmup step %d done" % (i + 1))
-    step_ms = []
-    corr_ms = []
-    query_ms = []
-    launches = 1
-    cands = 0
-    status = None
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        status, _, _, _ = eng.correct(0, n)
-        te = time.perf_counter()
-        st = eng.stats()
-        step_ms.append((te - ts) * 1e3)
-        corr_ms.append(st.correct_ms)
-        query_ms.append(st.query_ms)
-        cands = st.candidates
-        launches = max(1, st.launches)
-        log("bench: step %d %.1f ms (query %.1f ms, correct kernel %.1f ms)" % (i + 1, step_ms[-1], st.query_ms,
-                                                                             st.correct_ms))
-    ms = sum(step_ms) / len(step_ms)
-    value = n / (ms / 1e3)
+    plain = time_path(care_amd, rs, care_amd.default_options(**w["options"]), args.steps, args.warmup,
+                      batch=args.batch)
+    gated = None
+    gate_path = gate_mod.DEFAULT_GATE
+    gated_ok = args.workload == "repeat-pe" and os.path.exists(gate_path)
+    if gated_ok:
+        threshold = gate_mod.chosen_threshold(gate_path)
+        gopt = care_amd.default_options(**dict(w["options"], correction_type=3,
+                                                candidate_limit=gate_mod.CANDIDATE_LIMIT,
+                                                threshold_anchor=threshold))
+        gated = time_path(care_amd, rs, gopt, args.steps, args.warmup, gate=gate_mod.load(gate_path),
+                          batch=args.batch)
+    rs.close()
+
+    qual = None
+    if not args.no_quality and os.path.exists(truth):
+        work = os.path.join(os.path.dirname(reads), "quality")
+        qual = {"plain": quality(care_amd, reads, truth, w, [], os.path.join(work, "plain"))}
+        if gated_ok:
+            qual["gated"] = quality(care_amd, reads, truth, w, gate_mod._gated_args(gate_path, threshold),
+                                    os.path.join(work, "gated"))
+        log("bench: quality %s" % json.dumps(qual))
+
+    head = gated if gated is not None else plain
+    value = n / (head["ms"] / 1e3)
 
     # algorithmic bytes of the fused correction kernel per step: anchor and
     # candidate 2-bit rows, candidate lengths, candidate qualities, the
     # 16-byte per-candidate record written once, the corrected anchor out
-    pitch_bytes = rs.pitch * 4
-    qbytes = rs.qpitch * 4 if w["qual_bits"] else 0
-    alg_bytes = n * (pitch_bytes + rs.max_length + 1) + cands * (pitch_bytes + 4 + 16 + qbytes)
-    kern_ms = sum(corr_ms) / len(corr_ms)
-    launch_ms = kern_ms / launches
-    achieved = (alg_bytes / launches) / (launch_ms / 1e3) / 1e9
+    pitch_bytes = (w["max_length"] + 15) // 16 * 4
+    qbytes = ((w["qual_bits"] * w["max_length"] + 31) // 32) * 4 if w["qual_bits"] else 0
+
+    def roofline(r):
+        alg = n * (pitch_bytes + w["max_length"] + 1) + r["candidates"] * (pitch_bytes + 4 + 16 + qbytes)
+        launch_ms = r["correct_ms"] / r["launches"]
+        achieved = (alg / r["launches"]) / (launch_ms / 1e3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_correct<%s>" % ("true" if r is gated else "false"), "launches_per_step": r["launches"],
+                "bytes_per_launch": int(alg / r["launches"]), "avg_launch_ms": round(launch_ms, 4)}
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(reads, w, min(args.cpu_sample, n), threads)
 
+    def summary(r):
+        return {"reads_per_s": round(n / (r["ms"] / 1e3), 1), "ms_per_step": round(r["ms"], 3),
+                "query_kernel_ms": round(r["query_ms"], 3), "correct_kernel_ms": round(r["correct_ms"], 3),
+                "candidates_per_step": r["candidates"], "corrected_anchors": r["corrected"]}
+
+    config = {"workload": w["label"], "reads": n, "plain": summary(plain)}
+    if gated is not None:
+        config["mlp_gated"] = dict(summary(gated), threshold=threshold, candidate_limit=gate_mod.CANDIDATE_LIMIT,
+                                   method="top-%d candidates by map hits, no refinement, MLP gate on non-HQ "
+                                          "decision columns" % gate_mod.CANDIDATE_LIMIT)
+        config["speedup_gated_vs_plain"] = round(plain["ms"] / gated["ms"], 3)
+    if qual is not None:
+        config["quality"] = qual
     line = {
-        "metric": "reads corrected/s on 1 MI355X (plain classic path)",
+        "metric": "reads corrected/s on 1 MI355X: plain vs MLP-gated at equal base-level TP/FP"
+                  if gated is not None else "reads corrected/s on 1 MI355X (plain classic path)",
         "value": round(value, 1),
         "unit": "reads/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(head["ms"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i32+f32",
+        "dtype": "i32+f32+bf16" if gated is not None else "i32+f32",
         "data": "synthetic",
-        "config": {"workload": w["label"], "reads": n, "candidates_per_step": int(cands),
-                   "query_kernel_ms": round(sum(query_ms) / len(query_ms), 3),
-                   "correct_kernel_ms": round(kern_ms, 3),
-                   "corrected_anchors": int((status & 1).sum()) if status is not None else None},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_correct", "launches_per_step": launches,
-                     "bytes_per_launch": int(alg_bytes / launches), "avg_launch_ms": round(launch_ms, 4)},
+        "config": config,
+        "roofline": roofline(head),
         "cpu_baseline": cpu,
     }
-    eng.close()
+    if gated is not None:
+        line["roofline_plain"] = roofline(plain)
     print(json.dumps(line), flush=True)
 
 
