@@ -135,6 +135,7 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0):
 
 def quality(care_amd, reads, truth, w, extra, outdir):
     """Base-level TP/FP/FN of the care-gpu command line's output."""
+    os.makedirs(outdir, exist_ok=True)
     cmd = [care_amd.CARE_GPU, "-i", reads, "-o", "c.fastq", "-d", outdir, "-g", "0", "-t", "16"] + w["args"] + extra
     p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if p.returncode != 0:

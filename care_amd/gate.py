@@ -157,6 +157,7 @@ def select(gate_path, work):
     the fewest false positives among the thresholds that keep at least
     99.5% of the plain path's true positives (half of the allowed 1% loss is
     kept as margin for other read sets).  Stored beside the gate."""
+    os.makedirs(work, exist_ok=True)
     (vreads, vtruth), = simulate_many([VALIDATION_SEED], work, workers=1)
     plain = care_amd.evaluate(vreads, _run_cli(vreads, os.path.join(work, "vplain"), []), vtruth)
     sweep = {}

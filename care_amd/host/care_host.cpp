@@ -549,7 +549,9 @@ int runMain(int argc, char** argv) {
     if (o.eng.paired_end && o.inputs.size() > 2) throw std::runtime_error("paired-end input takes one or two files");
     if (o.eng.use_quality && o.qualityBits != 1 && o.qualityBits != 2 && o.qualityBits != 8)
         throw std::runtime_error("qualityScoreBits must be 1, 2 or 8");
-    mkdir(o.outdir.c_str(), 0755);
+    // output directory and its parents (std::filesystem::create_directories)
+    for (size_t i = 1; i <= o.outdir.size(); i++)
+        if (i == o.outdir.size() || o.outdir[i] == '/') mkdir(o.outdir.substr(0, i).c_str(), 0755);
     const int threads = std::max(1, o.threads);
 
     // STEP 1
