@@ -182,17 +182,28 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const bf16raw* X, const u8* Y,
         }
 }
 
-// Sums the partial gradients in a fixed order.  mode 0: writes the gradient
-// (and the loss at index kMlpParams) to out; mode 1: Adam step on the FP32
-// master weights (t = step number from 1), refreshes the BF16 copies and
-// adds the loss to *lossAcc.
-__global__ void k_mlp_reduce(const float* partial, int nPartials, int mode, float* out, float* master, float* m1,
-                             float* m2, float lr, float beta1, float beta2, float eps, float bc1, float bc2,
-                             bf16raw* w1b, bf16raw* w2b, bf16raw* w3b, float* lossAcc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > kMlpParams) return;
+// Sums the partial gradients in a fixed order: a block covers 32
+// parameters; its 8 thread groups sum the partials p = g, g + 8, ... of one
+// parameter each, and group sums are added in group order.  mode 0: writes
+// the gradient (and the loss at index kMlpParams) to out; mode 1: Adam step
+// on the FP32 master weights (t = step number from 1), refreshes the BF16
+// copies and adds the loss to *lossAcc.
+constexpr int kReduceParams = 32, kReduceGroups = 8;
+__global__ void __launch_bounds__(kReduceParams * kReduceGroups)
+    k_mlp_reduce(const float* partial, int nPartials, int mode, float* out, float* master, float* m1, float* m2,
+                 float lr, float beta1, float beta2, float eps, float bc1, float bc2, bf16raw* w1b, bf16raw* w2b,
+                 bf16raw* w3b, float* lossAcc) {
+    __shared__ float sPart[kReduceGroups][kReduceParams];
+    const int j = threadIdx.x % kReduceParams, grp = threadIdx.x / kReduceParams;
+    const int i = blockIdx.x * kReduceParams + j;
     float s = 0.f;
-    for (int p = 0; p < nPartials; p++) s += partial[u64(p) * kPartialStride + i];
+    if (i <= kMlpParams)
+        for (int p = grp; p < nPartials; p += kReduceGroups) s += partial[u64(p) * kPartialStride + i];
+    sPart[grp][j] = s;
+    __syncthreads();
+    if (grp != 0 || i > kMlpParams) return;
+    s = 0.f;
+    for (int g = 0; g < kReduceGroups; g++) s += sPart[g][j];
     if (mode == 0) {
         out[i] = s;
         return;
@@ -262,8 +273,9 @@ int mlp_partial_stride() { return kPartialStride; }
 void launch_mlp_reduce(const float* partial, int nPartials, int mode, float* out, float* master, float* m1, float* m2,
                        float lr, float beta1, float beta2, float eps, float bc1, float bc2, bf16raw* w1b, bf16raw* w2b,
                        bf16raw* w3b, float* lossAcc, hipStream_t s) {
-    const int blocks = (kMlpParams + 1 + 255) / 256;
-    hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks), dim3(256), 0, s, partial, nPartials, mode, out, master, m1, m2, lr,
+    const int blocks = (kMlpParams + 1 + kReduceParams - 1) / kReduceParams;
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks), dim3(kReduceParams * kReduceGroups), 0, s, partial, nPartials, mode,
+                       out, master, m1, m2, lr,
                        beta1, beta2, eps, bc1, bc2, w1b, w2b, w3b, lossAcc);
 }
 

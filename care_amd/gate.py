@@ -248,6 +248,25 @@ def evaluate(args):
     return res
 
 
+def timing(rows, epochs=1):
+    """Device training and inference throughput of the gate on synthetic
+    columns (a fixed linear rule), for kernel profiling."""
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0.0, 1.5, size=(rows, care_amd.MLP_FEATURES)).astype(np.float32)
+    y = (x[:, 0] + 0.5 * x[:, 3] > 0.9).astype(np.uint8)
+    with care_amd.Engine(care_amd.default_options(), device=0) as eng:
+        t = time.perf_counter()
+        p, losses = eng.mlp_train(x, y, he_init(1), epochs=epochs, batch=4096, lr=1e-3, seed=1)
+        dt = time.perf_counter() - t
+        t = time.perf_counter()
+        eng.mlp_infer(x, p)
+        di = time.perf_counter() - t
+    res = {"rows": rows, "epochs": epochs, "train_s": dt, "train_tflops": 3 * 9248.0 * rows * epochs / dt / 1e12,
+           "infer_s": di, "infer_tflops": 9248.0 * rows / di / 1e12, "loss": [float(v) for v in losses]}
+    print(json.dumps(res), flush=True)
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -260,6 +279,9 @@ def main(argv=None):
     t.add_argument("--workers", type=int, default=8)
     t.add_argument("--work", default=None)
     t.add_argument("--out", default=DEFAULT_GATE)
+    tm = sub.add_parser("timing")
+    tm.add_argument("--rows", type=int, default=4_000_000)
+    tm.add_argument("--epochs", type=int, default=1)
     v = sub.add_parser("select")
     v.add_argument("--gate", default=DEFAULT_GATE)
     v.add_argument("--work", default=None)
@@ -272,6 +294,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "train":
         train(a)
+    elif a.cmd == "timing":
+        timing(a.rows, a.epochs)
     elif a.cmd == "select":
         select(a.gate, a.work or tempfile.mkdtemp(prefix="care_gate_sel_"))
     else:
