@@ -44,7 +44,9 @@ def build_engine():
     os.makedirs(BUILD, exist_ok=True)
     srcs = [os.path.join(HERE, "csrc", s) for s in HIP_SOURCES]
     out = os.path.join(BUILD, "libcare_amd.so")
-    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", out + ".tmp"] + srcs)
+    # intra-library calls bind to this library even if another one exporting
+    # the same C-ABI is loaded into the process first
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-Wl,-Bsymbolic", "-o", out + ".tmp"] + srcs)
     os.replace(out + ".tmp", out)
     return out
 

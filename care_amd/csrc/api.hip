@@ -532,7 +532,18 @@ int care_minhash_query(care_context* ctx, uint64_t first, int n, uint32_t* ids, 
     return CARE_OK;
 }
 
-int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
+}  // extern "C"
+
+namespace {
+int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r);
+}
+
+extern "C" int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
+    return correct_batch_impl(ctx, first, n, r);
+}
+
+namespace {
+int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
     if (!ctx || !r || !r->anchor_status || !r->anchor_sequences) return fail(CARE_ERR_ARGUMENT, "null argument");
     if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
     if (n < 0 || first + uint64_t(n) > ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "range outside read store");
@@ -685,6 +696,10 @@ int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_resu
     }
     return CARE_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int care_last_batch_stats(care_context* ctx, care_batch_stats* stats) {
     if (!ctx || !stats) return fail(CARE_ERR_ARGUMENT, "null argument");
@@ -1045,7 +1060,7 @@ int care_gate_columns(care_context* ctx, uint64_t first, int n, float* features,
     res.sequence_pitch = pitch;
     const bool cc = ctx->opt.correct_candidates != 0;
     ctx->opt.correct_candidates = 0;
-    const int rc = care_correct_batch(ctx, first, n, &res);
+    const int rc = correct_batch_impl(ctx, first, n, &res);
     ctx->opt.correct_candidates = cc;
     r.on = false;
     r.cap = savedCap;

@@ -8,7 +8,7 @@ here="$(cd "$(dirname "$0")" && pwd)"
 root="$(dirname "$(dirname "$here")")"
 out="$here/_build"
 mkdir -p "$out"
-FLAGS="-std=c++20 -O2 -g -U_FORTIFY_SOURCE -fPIC -pthread -ffp-contract=off -w -I$here"
+FLAGS="-std=c++20 -O2 -g -U_FORTIFY_SOURCE -fPIC -pthread -ffp-contract=off -w -I$here -Wl,-Bsymbolic"
 g++ $FLAGS -shared -x c++ -o "$out/libcare_amd_emu.so" \
     "$root/care_amd/csrc/api.hip" "$root/care_amd/csrc/minhash.hip" "$root/care_amd/csrc/correct.hip" \
     "$root/care_amd/csrc/mlp.hip"
