@@ -23,6 +23,8 @@ build runs on one device.
 
 import argparse
 import json
+
+import numpy as np
 import os
 import re
 import subprocess
@@ -112,14 +114,19 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0):
     if gate is not None:
         eng.mlp_load(gate)
     n = rs.n
+    # result arrays allocated once and reused by every step, as a caller
+    # correcting batch after batch would
+    pitch = (rs.max_length + 15) // 16 * 16
+    out = (np.zeros(n, dtype=np.uint8), np.zeros((n, pitch), dtype=np.uint8))
+    out[1].fill(0)
     for i in range(warmup):
-        eng.correct(0, n)
+        eng.correct(0, n, out=out)
     step_ms, corr_ms, query_ms = [], [], []
     st = None
     status = None
     for i in range(steps):
         ts = time.perf_counter()
-        status, _, _, _ = eng.correct(0, n)
+        status, _, _, _ = eng.correct(0, n, out=out)
         te = time.perf_counter()
         st = eng.stats()
         step_ms.append((te - ts) * 1e3)

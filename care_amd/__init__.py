@@ -309,10 +309,16 @@ class Engine:
             self._ck(rc)
             return ids[: offs[n]], offs
 
-    def correct(self, first, n, candidate_capacity=None):
+    def correct(self, first, n, candidate_capacity=None, out=None):
+        """Corrects anchors [first, first+n).  out: (status, seqs) arrays to
+        reuse across calls (uint8 [n] and uint8 [n][pitch])."""
         pitch = (self.max_length + 15) // 16 * 16
-        status = np.zeros(n, dtype=np.uint8)
-        seqs = np.zeros((n, pitch), dtype=np.uint8)
+        if out is not None:
+            status, seqs = out
+            assert status.shape[0] >= n and seqs.shape[0] >= n and seqs.shape[1] == pitch
+        else:
+            status = np.zeros(n, dtype=np.uint8)
+            seqs = np.zeros((n, pitch), dtype=np.uint8)
         ccap = candidate_capacity if candidate_capacity is not None else (n * 64 if self.options.correct_candidates else 0)
         cids = np.zeros(max(ccap, 1), dtype=np.uint32)
         cseqs = np.zeros((max(ccap, 1), pitch), dtype=np.uint8)

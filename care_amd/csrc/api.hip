@@ -125,6 +125,7 @@ struct care_context {
     bool hasReads = false;
 
     // tables
+    DevBuf<u64> sigs;  // [n][h] signatures of all reads
     std::vector<std::unique_ptr<MapStore>> maps;
     DevBuf<DevMap> devMaps;
     DevTables tables{};
@@ -142,16 +143,14 @@ struct care_context {
     DevBuf<u8> outStatus;
     DevBuf<char> outSeq;
     int outPitch = 0;
+    int outCap = 0;                     // anchors the output buffers hold
+    std::vector<hipEvent_t> batchEv;    // 3 per device batch of a call
     DevBuf<u32> candOutIds;
     DevBuf<char> candOutSeq;
     int candOutCap = 0;
     DevBuf<int> counters;  // [0] queue head, [1] candidate output count
     DevBuf<unsigned long long> rawHits;
     DevBuf<float> qweights;
-    PinnedBuf<u8> hStatus;
-    PinnedBuf<char> hSeq;
-    PinnedBuf<u32> hCandIds;
-    PinnedBuf<char> hCandSeq;
     PinnedBuf<int> hInts;
     PinnedBuf<unsigned long long> hU64;
 
@@ -194,16 +193,13 @@ void ensure_batch(care_context* ctx, int batch) {
     ctx->outStatus.alloc(batch);
     ctx->outPitch = (ctx->rs.maxLen + 15) / 16 * 16;
     ctx->outSeq.alloc(size_t(batch) * ctx->outPitch);
-    ctx->hStatus.alloc(batch);
-    ctx->hSeq.alloc(size_t(batch) * ctx->outPitch);
+    ctx->outCap = batch;
     if (ctx->opt.correct_candidates) {
         // every HQ anchor corrects at most its candidates within the window;
         // a generous average bound, checked per batch
         ctx->candOutCap = int(std::min<size_t>(size_t(batch) * 64, size_t(1) << 26));
         ctx->candOutIds.alloc(ctx->candOutCap);
         ctx->candOutSeq.alloc(size_t(ctx->candOutCap) * ctx->outPitch);
-        ctx->hCandIds.alloc(ctx->candOutCap);
-        ctx->hCandSeq.alloc(size_t(ctx->candOutCap) * ctx->outPitch);
     }
     ctx->batchCap = batch;
 }
@@ -313,6 +309,7 @@ int care_context_destroy(care_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->batchEv) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CARE_OK;
@@ -430,7 +427,7 @@ int care_minhash_build(care_context* ctx) {
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
-        DevBuf<u64> sigs;
+        DevBuf<u64>& sigs = ctx->sigs;  // kept: queries read the anchors' signatures
         sigs.alloc(n * h);
         launch_signatures(ctx->rs, 0, int(n), ctx->opt.kmer_length, h, sigs.p, s);
         HIP_TRY(hipGetLastError());
@@ -489,6 +486,7 @@ int care_minhash_build(care_context* ctx) {
     ctx->tables.h = h;
     ctx->tables.k = ctx->opt.kmer_length;
     ctx->tables.kmask = ~u64(0) >> (2 * (32 - ctx->opt.kmer_length));
+    ctx->tables.sigs = ctx->sigs.p;
     ctx->hasTables = true;
     return CARE_OK;
 }
@@ -536,6 +534,18 @@ int care_minhash_query(care_context* ctx, uint64_t first, int n, uint32_t* ids, 
 
 namespace {
 int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r);
+
+// rows of `width` bytes from a device buffer with pitch spitch to host rows
+// with pitch dpitch: one contiguous copy when the pitches agree
+void copy_rows_to_host(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t rows,
+                       hipStream_t s) {
+    if (rows == 0) return;
+    if (dpitch == spitch) {
+        HIP_TRY(hipMemcpyAsync(dst, src, (rows - 1) * spitch + width, hipMemcpyDeviceToHost, s));
+    } else {
+        HIP_TRY(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyDeviceToHost, s));
+    }
+}
 }
 
 extern "C" int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
@@ -632,60 +642,82 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
         P.candOutCount = ctx->counters.p + 1;
         P.candOutCap = ctx->candOutCap;
         P.queueHead = ctx->counters.p;
-        float qms = 0, cms = 0;
+        // Without candidate correction every device batch writes its own
+        // slice of call-wide output buffers, so batches run back to back with
+        // no host synchronisation and the results come back in two copies.
+        // With candidate correction the variable-size candidate output is
+        // collected after every batch.
+        const bool async = !o.correct_candidates;
+        const int nBatches = (n + B - 1) / B;
+        if (async && n > ctx->outCap) {
+            ctx->outStatus.alloc(size_t(n));
+            ctx->outSeq.alloc(size_t(n) * ctx->outPitch);
+            ctx->outCap = n;
+        }
+        while (int(ctx->batchEv.size()) < 3 * nBatches) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            ctx->batchEv.push_back(e);
+        }
+        launch_fill_u64(reinterpret_cast<u64*>(ctx->rawHits.p), 2, 0, s);  // totals over the call
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
-        for (int done = 0; done < n; done += B) {
+        int bi = 0;
+        for (int done = 0; done < n; done += B, bi++) {
             const int nb = std::min(B, n - done);
-            launch_fill_u64(reinterpret_cast<u64*>(ctx->rawHits.p), 2, 0, s);
-            HIP_TRY(hipEventRecord(ctx->ev[0], s));
+            hipEvent_t* e = &ctx->batchEv[size_t(3 * bi)];
+            HIP_TRY(hipEventRecord(e[0], s));
             launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, o.exclude_ambiguous,
                          o.correction_type == 3 ? o.candidate_limit : 0, ctx->candIds.p,
                          ctx->candCount.p, ctx->rawHits.p, s);
-            HIP_TRY(hipEventRecord(ctx->ev[1], s));
+            HIP_TRY(hipEventRecord(e[1], s));
             launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);
             P.firstAnchor = first + done;
             P.numAnchors = nb;
+            P.outStatus = ctx->outStatus.p + (async ? size_t(done) : 0);
+            P.outSeq = ctx->outSeq.p + (async ? size_t(done) * ctx->outPitch : 0);
             const int blocks = std::max(1, std::min(nb, ctx->numCUs * blocksPerCU));
             launch_correct(P, blocks, T, s);
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipEventRecord(ctx->ev[3], s));
-            HIP_TRY(hipMemcpyAsync(ctx->hStatus.p, ctx->outStatus.p, nb, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(ctx->hSeq.p, ctx->outSeq.p, size_t(nb) * ctx->outPitch, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(ctx->hInts.p, ctx->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(ctx->hU64.p, ctx->rawHits.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
-            float t1 = 0, t2 = 0;
-            HIP_TRY(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
-            HIP_TRY(hipEventElapsedTime(&t2, ctx->ev[1], ctx->ev[3]));
-            qms += t1;
-            cms += t2;
+            HIP_TRY(hipEventRecord(e[2], s));
             ctx->stats.launches++;
-            ctx->stats.raw_hits += ctx->hU64.p[0];
-            ctx->stats.candidates += ctx->hU64.p[1];
-            for (int i = 0; i < nb; i++) {
-                r->anchor_status[done + i] = ctx->hStatus.p[i];
-                if (ctx->hStatus.p[i] & 1)
-                    std::memcpy(r->anchor_sequences + size_t(done + i) * r->sequence_pitch, ctx->hSeq.p + size_t(i) * ctx->outPitch,
-                                ctx->rs.maxLen);
-            }
-            if (o.correct_candidates) {
-                const int nc = ctx->hInts.p[1];
-                if (nc > ctx->candOutCap) return fail(CARE_ERR_CAPACITY, "candidate correction buffer overflow (%d)", nc);
-                if (r->num_candidates + nc > r->candidate_capacity)
-                    return fail(CARE_ERR_CAPACITY, "candidate_capacity exceeded");
-                HIP_TRY(hipMemcpyAsync(ctx->hCandIds.p, ctx->candOutIds.p, sizeof(u32) * nc, hipMemcpyDeviceToHost, s));
-                HIP_TRY(hipMemcpyAsync(ctx->hCandSeq.p, ctx->candOutSeq.p, size_t(nc) * ctx->outPitch, hipMemcpyDeviceToHost, s));
+            if (async) continue;
+            HIP_TRY(hipMemcpyAsync(r->anchor_status + done, ctx->outStatus.p, size_t(nb), hipMemcpyDeviceToHost, s));
+            copy_rows_to_host(r->anchor_sequences + size_t(done) * r->sequence_pitch, size_t(r->sequence_pitch),
+                              ctx->outSeq.p, size_t(ctx->outPitch), size_t(ctx->rs.maxLen), size_t(nb), s);
+            HIP_TRY(hipMemcpyAsync(ctx->hInts.p, ctx->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            const int nc = ctx->hInts.p[1];
+            if (nc > ctx->candOutCap) return fail(CARE_ERR_CAPACITY, "candidate correction buffer overflow (%d)", nc);
+            if (r->num_candidates + nc > r->candidate_capacity)
+                return fail(CARE_ERR_CAPACITY, "candidate_capacity exceeded");
+            if (nc > 0) {
+                HIP_TRY(hipMemcpyAsync(r->candidate_read_ids + r->num_candidates, ctx->candOutIds.p, sizeof(u32) * nc,
+                                       hipMemcpyDeviceToHost, s));
+                copy_rows_to_host(r->candidate_sequences + size_t(r->num_candidates) * r->sequence_pitch,
+                                  size_t(r->sequence_pitch), ctx->candOutSeq.p, size_t(ctx->outPitch),
+                                  size_t(ctx->rs.maxLen), size_t(nc), s);
                 HIP_TRY(hipStreamSynchronize(s));
-                for (int i = 0; i < nc; i++) {
-                    r->candidate_read_ids[r->num_candidates + i] = ctx->hCandIds.p[i];
-                    std::memcpy(r->candidate_sequences + size_t(r->num_candidates + i) * r->sequence_pitch,
-                                ctx->hCandSeq.p + size_t(i) * ctx->outPitch, ctx->rs.maxLen);
-                }
-                r->num_candidates += nc;
             }
+            r->num_candidates += nc;
         }
+        if (async) {
+            HIP_TRY(hipMemcpyAsync(r->anchor_status, ctx->outStatus.p, size_t(n), hipMemcpyDeviceToHost, s));
+            copy_rows_to_host(r->anchor_sequences, size_t(r->sequence_pitch), ctx->outSeq.p, size_t(ctx->outPitch),
+                              size_t(ctx->rs.maxLen), size_t(n), s);
+        }
+        HIP_TRY(hipMemcpyAsync(ctx->hU64.p, ctx->rawHits.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipEventRecord(ctx->ev[3], s));
         HIP_TRY(hipEventSynchronize(ctx->ev[3]));
+        float qms = 0, cms = 0;
+        for (int i = 0; i < bi; i++) {
+            float t1 = 0, t2 = 0;
+            HIP_TRY(hipEventElapsedTime(&t1, ctx->batchEv[size_t(3 * i)], ctx->batchEv[size_t(3 * i + 1)]));
+            HIP_TRY(hipEventElapsedTime(&t2, ctx->batchEv[size_t(3 * i + 1)], ctx->batchEv[size_t(3 * i + 2)]));
+            qms += t1;
+            cms += t2;
+        }
+        ctx->stats.raw_hits = ctx->hU64.p[0];
+        ctx->stats.candidates = ctx->hU64.p[1];
         float tot = 0;
         HIP_TRY(hipEventElapsedTime(&tot, ctx->ev[2], ctx->ev[3]));
         ctx->stats.query_ms = qms;

@@ -43,6 +43,7 @@ struct DevTables {
     int h = 0;
     int k = 0;
     u64 kmask = 0;
+    const u64* sigs = nullptr;  // [n][h] signatures of the read store, kept from the build
 };
 
 // Per-candidate record of the correction kernel (16 bytes).

@@ -287,7 +287,13 @@ __global__ void k_query(DevReadStore rs, DevTables t, u64 first, int n, int maxC
             if (threadIdx.x == 0) candCount[a] = 0;
             continue;  // uniform per block
         }
-        block_signature(rs.seqs + id * rs.pitch, len, t.k, t.h, canon, partial, sig);
+        if (t.sigs) {
+            // signatures computed once for the table build
+            for (int j = threadIdx.x; j < t.h; j += blockDim.x) sig[j] = t.sigs[id * u64(t.h) + j];
+            __syncthreads();
+        } else {
+            block_signature(rs.seqs + id * rs.pitch, len, t.k, t.h, canon, partial, sig);
+        }
         // probe every map
         for (int j = threadIdx.x; j < t.h; j += blockDim.x) {
             const DevMap m = t.maps[j];

@@ -108,7 +108,8 @@ int care_minhash_query(care_context* ctx, uint64_t first, int n, uint32_t* ids, 
 typedef struct care_batch_result {
     /* per anchor: bit0 = corrected, bit1 = high-quality MSA */
     uint8_t* anchor_status;      /* [num_anchors] */
-    char* anchor_sequences;      /* [num_anchors * sequence_pitch] corrected bases */
+    char* anchor_sequences;      /* [num_anchors * sequence_pitch] corrected bases (first max_length
+                                    bytes of a row; rows of uncorrected anchors are unspecified) */
     int sequence_pitch;          /* bytes per row, >= max_length */
     /* candidate corrections (only with correct_candidates) */
     uint32_t* candidate_read_ids;  /* [candidate_capacity] */

@@ -495,6 +495,12 @@ inline hipError_t hipHostFree(void* p) {
     std::free(p);
     return hipSuccess;
 }
+inline hipError_t hipMemcpy2DAsync(void* d, size_t dpitch, const void* s, size_t spitch, size_t width,
+                                   size_t height, hipMemcpyKind, hipStream_t = nullptr) {
+    for (size_t r = 0; r < height; r++)
+        std::memcpy(static_cast<char*>(d) + r * dpitch, static_cast<const char*>(s) + r * spitch, width);
+    return hipSuccess;
+}
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
     std::memcpy(d, s, n);
     return hipSuccess;
