@@ -102,7 +102,7 @@ def cpu_baseline(reads, w, sample, threads):
                       "query+correction %.2f s on %d threads, %s" % (sample, float(m.group(1)), threads, model)}
 
 
-def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0):
+def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=True):
     """Average wall time of one correction step over all reads (reads and
     tables resident in HBM), the correction kernel's HIP-event time and
     launch count, and the candidates of the last step."""
@@ -114,6 +114,9 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0):
     if gate is not None:
         eng.mlp_load(gate)
     n = rs.n
+    # launch shapes timed on the first anchors (set-up, like the table build)
+    tuning = eng.autotune(0, min(n, 65536)) if autotune else eng.tuning()
+    log("bench: %s launch shapes %s" % ("gated" if gate is not None else "plain", json.dumps(tuning)))
     # result arrays allocated once and reused by every step, as a caller
     # correcting batch after batch would
     pitch = (rs.max_length + 15) // 16 * 16
@@ -137,6 +140,7 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0):
     eng.close()
     return {"ms": sum(step_ms) / len(step_ms), "correct_ms": sum(corr_ms) / len(corr_ms),
             "query_ms": sum(query_ms) / len(query_ms), "launches": max(1, st.launches), "candidates": int(st.candidates),
+            "raw_hits": int(st.raw_hits), "h": int(opt.num_hash_maps), "tuning": tuning,
             "corrected": int((status & 1).sum())}
 
 
@@ -163,6 +167,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -181,7 +186,7 @@ def main():
     log("bench: %d reads packed in %.2f s" % (n, time.perf_counter() - t0))
 
     plain = time_path(care_amd, rs, care_amd.default_options(**w["options"]), args.steps, args.warmup,
-                      batch=args.batch)
+                      batch=args.batch, autotune=not args.no_autotune)
     gated = None
     gate_path = gate_mod.DEFAULT_GATE
     gated_ok = args.workload == "repeat-pe" and os.path.exists(gate_path)
@@ -191,7 +196,7 @@ def main():
                                                 candidate_limit=gate_mod.CANDIDATE_LIMIT,
                                                 threshold_anchor=threshold))
         gated = time_path(care_amd, rs, gopt, args.steps, args.warmup, gate=gate_mod.load(gate_path),
-                          batch=args.batch)
+                          batch=args.batch, autotune=not args.no_autotune)
     rs.close()
 
     qual = None
@@ -221,6 +226,16 @@ def main():
                 "kernel": "k_correct<%s>" % ("true" if r is gated else "false"), "launches_per_step": r["launches"],
                 "bytes_per_launch": int(alg / r["launches"]), "avg_launch_ms": round(launch_ms, 4)}
 
+    def roofline_query(r):
+        # per anchor: h stored signatures, h probes of (key, offset/count),
+        # the hit values, candidate ids and the count written
+        alg = n * (r["h"] * (8 + 8 + 8) + 4) + 4 * r["raw_hits"] + 4 * r["candidates"]
+        launch_ms = r["query_ms"] / r["launches"]
+        achieved = (alg / r["launches"]) / (launch_ms / 1e3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "kernel": "k_query", "raw_hits_per_step": r["raw_hits"],
+                "bytes_per_launch": int(alg / r["launches"]), "avg_launch_ms": round(launch_ms, 4)}
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -229,7 +244,7 @@ def main():
     def summary(r):
         return {"reads_per_s": round(n / (r["ms"] / 1e3), 1), "ms_per_step": round(r["ms"], 3),
                 "query_kernel_ms": round(r["query_ms"], 3), "correct_kernel_ms": round(r["correct_ms"], 3),
-                "candidates_per_step": r["candidates"], "corrected_anchors": r["corrected"]}
+                "candidates_per_step": r["candidates"], "corrected_anchors": r["corrected"], "launch_shapes": r["tuning"]}
 
     config = {"workload": w["label"], "reads": n, "plain": summary(plain)}
     if gated is not None:
@@ -259,6 +274,7 @@ def main():
     }
     if gated is not None:
         line["roofline_plain"] = roofline(plain)
+    line["roofline_query"] = roofline_query(plain)
     print(json.dumps(line), flush=True)
 
 

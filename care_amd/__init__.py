@@ -74,6 +74,22 @@ class BatchStats(ctypes.Structure):
     ]
 
 
+class CareTuning(ctypes.Structure):
+    """Launch shapes of the query and correction kernels (care_tuning);
+    no shape changes a result."""
+    _fields_ = [
+        ("batch_anchors", ctypes.c_int),
+        ("correct_threads", ctypes.c_int),
+        ("correct_blocks_per_cu", ctypes.c_int),
+        ("msa_chunk", ctypes.c_int),
+        ("query_lds_hits", ctypes.c_int),
+        ("query_register_hits", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 MLP_FEATURES = 32
 MLP_PARAMS = 4737
 
@@ -108,6 +124,9 @@ ENGINE_SYMBOLS = {
     "care_correct_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
                                           ctypes.POINTER(BatchResult)]),
     "care_set_launch_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "care_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CareTuning)]),
+    "care_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CareTuning)]),
+    "care_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(CareTuning)]),
     "care_forest_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint32]),
     "care_last_batch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchStats)]),
     "care_mlp_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -291,6 +310,22 @@ class Engine:
 
     def set_launch_config(self, batch=0, threads=0):
         self._ck(self.lib.care_set_launch_config(self.ctx, batch, threads))
+
+    def set_tuning(self, **shape):
+        """Sets launch shapes (care_tuning fields; missing fields are 0 = automatic)."""
+        t = CareTuning(**shape)
+        self._ck(self.lib.care_set_tuning(self.ctx, ctypes.byref(t)))
+
+    def tuning(self):
+        t = CareTuning()
+        self._ck(self.lib.care_get_tuning(self.ctx, ctypes.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, first=0, n=65536):
+        """Times candidate launch shapes on anchors [first, first+n) and keeps the fastest."""
+        t = CareTuning()
+        self._ck(self.lib.care_autotune(self.ctx, first, n, ctypes.byref(t)))
+        return t.as_dict()
 
     def signatures(self, first, n):
         out = np.zeros((n, self.options.num_hash_maps), dtype=np.uint64)

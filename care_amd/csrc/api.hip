@@ -135,10 +135,15 @@ struct care_context {
     // batch buffers
     int batchCap = 0;
     int maxCand = 0;
-    int blockThreads = 256;
+    int queryCap = 0;  // hit lists sorted in LDS up to this length, longer ones in global scratch
+    int queryRegMax = 0;  // hit lists sorted in registers up to this length (0 = kernel maximum)
+    int blockThreads = 256;  // correction workgroup
+    int blocksPerCU = 0;     // persistent correction workgroups per CU (0 = resident maximum)
+    int msaChunk = 0;        // candidates per MSA staging pass (0 = automatic)
     int requestedBatch = 0;
     DevBuf<u32> candIds;
     DevBuf<int> candCount;
+    DevBuf<int> rawCount;  // table hits per anchor of a batch (statistics)
     DevBuf<CandMeta> metaA, metaB;
     DevBuf<u8> outStatus;
     DevBuf<char> outSeq;
@@ -188,6 +193,7 @@ void ensure_batch(care_context* ctx, int batch) {
     const size_t mc = size_t(ctx->maxCand);
     ctx->candIds.alloc(size_t(batch) * mc);
     ctx->candCount.alloc(batch);
+    ctx->rawCount.alloc(batch);
     ctx->metaA.alloc(size_t(batch) * mc);
     ctx->metaB.alloc(size_t(batch) * mc);
     ctx->outStatus.alloc(batch);
@@ -279,6 +285,7 @@ int care_context_create(int device, const care_options* opt, care_context** out)
         HIP_TRY(hipDeviceGetAttribute(&ctx->numCUs, hipDeviceAttributeMultiprocessorCount, device));
         ctx->cap = resultsPerMapThreshold(opt->coverage);
         ctx->maxCand = opt->num_hash_maps * ctx->cap;
+        ctx->queryCap = std::min((ctx->maxCand + 15) / 16 * 16, kDefaultQueryCap);
         // quality weights: include/qualityscoreweights.hpp:15-30 (host powf)
         float w[256];
         for (int i = 0; i < 256; i++) {
@@ -317,12 +324,60 @@ int care_context_destroy(care_context* ctx) {
 
 int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threads) {
     if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
-    if (batch_anchors < 0 || (block_threads != 0 && block_threads != 64 && block_threads != 128 &&
-                              block_threads != 256 && block_threads != 512))
-        return fail(CARE_ERR_ARGUMENT, "invalid launch configuration");
+    if (batch_anchors < 0 || (block_threads != 0 && block_threads != 128 && block_threads != 256))
+        return fail(CARE_ERR_ARGUMENT, "invalid launch configuration (block_threads 128 or 256)");
     if (batch_anchors > 0 && (batch_anchors & 1)) return fail(CARE_ERR_ARGUMENT, "batch_anchors must be even");
     ctx->requestedBatch = batch_anchors;
-    ctx->blockThreads = block_threads ? block_threads : 256;
+    if (block_threads) ctx->blockThreads = block_threads;
+    return CARE_OK;
+}
+
+int care_set_tuning(care_context* ctx, const care_tuning* t) {
+    if (!ctx || !t) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (t->batch_anchors < 0 || (t->batch_anchors & 1)) return fail(CARE_ERR_ARGUMENT, "batch_anchors must be even and >= 0");
+    if (t->correct_threads != 0 && t->correct_threads != 128 && t->correct_threads != 256)
+        return fail(CARE_ERR_ARGUMENT, "correct_threads must be 128 or 256");
+    if (t->correct_blocks_per_cu < 0 || t->correct_blocks_per_cu > 64)
+        return fail(CARE_ERR_ARGUMENT, "correct_blocks_per_cu outside 0..64");
+    if (t->msa_chunk != 0 && (t->msa_chunk < 16 || t->msa_chunk > 256))
+        return fail(CARE_ERR_ARGUMENT, "msa_chunk outside 16..256");
+    const int q = t->query_lds_hits ? t->query_lds_hits : kDefaultQueryCap;
+    if (q < 16 || q > 4096 || (q & 15)) return fail(CARE_ERR_ARGUMENT, "query_lds_hits must be a multiple of 16 in 16..4096");
+    if (4 * query_wave_lds_bytes(q, ctx->opt.candidate_limit) > size_t(kMaxLdsBytes))
+        return fail(CARE_ERR_CAPACITY, "query_lds_hits %d exceeds the query workgroup's LDS", q);
+    const int rm = t->query_register_hits;
+    if (rm != 0 && (rm < 64 || rm > 1024 || (rm & (rm - 1))))
+        return fail(CARE_ERR_ARGUMENT, "query_register_hits must be 0 or a power of two in 64..1024");
+    ctx->requestedBatch = t->batch_anchors;
+    ctx->blockThreads = t->correct_threads ? t->correct_threads : 256;
+    ctx->blocksPerCU = t->correct_blocks_per_cu;
+    ctx->msaChunk = t->msa_chunk;
+    ctx->queryCap = q;
+    ctx->queryRegMax = rm;
+    return CARE_OK;
+}
+
+int care_get_tuning(care_context* ctx, care_tuning* out) {
+    if (!ctx || !out) return fail(CARE_ERR_ARGUMENT, "null argument");
+    const auto& o = ctx->opt;
+    const int T = ctx->blockThreads;
+    const int gatedK = o.correction_type == 3;
+    out->batch_anchors = ctx->rs.n ? default_batch(ctx) : ctx->requestedBatch;
+    out->correct_threads = T;
+    out->msa_chunk = correct_msa_chunk(ctx->rs.maxLen ? ctx->rs.maxLen : 1, o.use_quality, ctx->rs.qpitch, T, gatedK,
+                                       ctx->msaChunk);
+    out->query_lds_hits = ctx->queryCap;
+    out->query_register_hits = ctx->queryRegMax ? ctx->queryRegMax : 1024;
+    out->correct_blocks_per_cu = ctx->blocksPerCU;
+    if (ctx->blocksPerCU == 0 && ctx->rs.maxLen) {
+        try {
+            HIP_TRY(hipSetDevice(ctx->device));
+            const size_t lds = correct_lds_bytes(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T, gatedK, ctx->msaChunk);
+            out->correct_blocks_per_cu = correct_resident_blocks(T, lds, gatedK);
+        } catch (const HipError& e) {
+            return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+        }
+    }
     return CARE_OK;
 }
 
@@ -424,6 +479,7 @@ int care_minhash_build(care_context* ctx) {
     if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");
     const int h = ctx->opt.num_hash_maps;
     const u64 n = ctx->rs.n;
+    // one query workgroup holds up to h * cap hits twice in LDS
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
@@ -506,9 +562,10 @@ int care_minhash_query(care_context* ctx, uint64_t first, int n, uint32_t* ids, 
         uint64_t written = 0;
         for (int done = 0; done < n; done += B) {
             const int nb = std::min(B, n - done);
-            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->opt.exclude_ambiguous,
-                         ctx->opt.correction_type == 3 ? ctx->opt.candidate_limit : 0,
-                         ctx->candIds.p, ctx->candCount.p, nullptr, ctx->stream);
+            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->queryCap, ctx->queryRegMax,
+                         ctx->opt.exclude_ambiguous, ctx->opt.correction_type == 3 ? ctx->opt.candidate_limit : 0,
+                         ctx->candIds.p, ctx->candCount.p, reinterpret_cast<u32*>(ctx->metaA.p), nullptr,
+                         ctx->stream);
             HIP_TRY(hipGetLastError());
             counts.resize(nb);
             buf.resize(size_t(nb) * ctx->maxCand);
@@ -553,19 +610,10 @@ extern "C" int care_correct_batch(care_context* ctx, uint64_t first, int n, care
 }
 
 namespace {
-int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
-    if (!ctx || !r || !r->anchor_status || !r->anchor_sequences) return fail(CARE_ERR_ARGUMENT, "null argument");
-    if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
-    if (n < 0 || first + uint64_t(n) > ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "range outside read store");
-    if (r->sequence_pitch < ctx->rs.maxLen) return fail(CARE_ERR_ARGUMENT, "sequence_pitch < max read length");
-    if (ctx->opt.paired_end && ((first & 1) || (n & 1)))
-        return fail(CARE_ERR_ARGUMENT, "paired-end batches must start at an even id and hold whole pairs");
-    if (ctx->opt.correct_candidates && (!r->candidate_read_ids || !r->candidate_sequences))
-        return fail(CARE_ERR_ARGUMENT, "candidate output buffers required with correct_candidates");
-    r->num_candidates = 0;
-    ctx->stats = care_batch_stats{};
+// Scalar parameters and model pointers of the correction kernel from the
+// context's options (device buffers are filled in by the caller).
+int prepare_params(care_context* ctx, CorrectParams& P) {
     const auto& o = ctx->opt;
-    CorrectParams P{};
     P.rs = ctx->rs;
     P.maxCand = ctx->maxCand;
     P.qweights = ctx->qweights.p;
@@ -621,14 +669,34 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
         if (!ctx->forest[1].loaded) return fail(CARE_ERR_STATE, "correction_type_cands 1 needs a candidate forest (care_forest_load)");
         P.forestC = ctx->forest[1].dev;
     }
+    return CARE_OK;
+}
+
+int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
+    if (!ctx || !r || !r->anchor_status || !r->anchor_sequences) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
+    if (n < 0 || first + uint64_t(n) > ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "range outside read store");
+    if (r->sequence_pitch < ctx->rs.maxLen) return fail(CARE_ERR_ARGUMENT, "sequence_pitch < max read length");
+    if (ctx->opt.paired_end && ((first & 1) || (n & 1)))
+        return fail(CARE_ERR_ARGUMENT, "paired-end batches must start at an even id and hold whole pairs");
+    if (ctx->opt.correct_candidates && (!r->candidate_read_ids || !r->candidate_sequences))
+        return fail(CARE_ERR_ARGUMENT, "candidate output buffers required with correct_candidates");
+    r->num_candidates = 0;
+    ctx->stats = care_batch_stats{};
+    const auto& o = ctx->opt;
+    CorrectParams P{};
+    if (const int rc = prepare_params(ctx, P)) return rc;
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         const int B = default_batch(ctx);
         ensure_batch(ctx, B);
         const int T = ctx->blockThreads;
-        int blocksPerCU = 0;
-        const size_t lds = correct_lds_bytes(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T, o.correction_type == 3);
-        blocksPerCU = int(std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds + 1024))));
+        const int gatedK = o.correction_type == 3;
+        const size_t lds = correct_lds_bytes(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T, gatedK, ctx->msaChunk);
+        if (lds > size_t(kMaxLdsBytes))
+            return fail(CARE_ERR_CAPACITY, "reads of length %d need %zu bytes of LDS per correction workgroup (limit %d)",
+                        ctx->rs.maxLen, lds, kMaxLdsBytes);
+        const int blocksPerCU = ctx->blocksPerCU > 0 ? ctx->blocksPerCU : correct_resident_blocks(T, lds, gatedK);
         hipStream_t s = ctx->stream;
         P.candIds = ctx->candIds.p;
         P.candCount = ctx->candCount.p;
@@ -666,9 +734,11 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
             const int nb = std::min(B, n - done);
             hipEvent_t* e = &ctx->batchEv[size_t(3 * bi)];
             HIP_TRY(hipEventRecord(e[0], s));
-            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, o.exclude_ambiguous,
-                         o.correction_type == 3 ? o.candidate_limit : 0, ctx->candIds.p,
-                         ctx->candCount.p, ctx->rawHits.p, s);
+            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->queryCap, ctx->queryRegMax,
+                         o.exclude_ambiguous,
+                         o.correction_type == 3 ? o.candidate_limit : 0, ctx->candIds.p, ctx->candCount.p,
+                         reinterpret_cast<u32*>(ctx->metaA.p), ctx->rawCount.p, s);
+            launch_sum_counts(ctx->rawCount.p, ctx->candCount.p, nb, ctx->rawHits.p, s);
             HIP_TRY(hipEventRecord(e[1], s));
             launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);
             P.firstAnchor = first + done;
@@ -676,7 +746,7 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
             P.outStatus = ctx->outStatus.p + (async ? size_t(done) : 0);
             P.outSeq = ctx->outSeq.p + (async ? size_t(done) * ctx->outPitch : 0);
             const int blocks = std::max(1, std::min(nb, ctx->numCUs * blocksPerCU));
-            launch_correct(P, blocks, T, s);
+            launch_correct(P, blocks, T, ctx->msaChunk, s);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipEventRecord(e[2], s));
             ctx->stats.launches++;
@@ -878,6 +948,7 @@ int care_mlp_load(care_context* ctx, const float* params) {
     return CARE_OK;
 }
 
+extern "C++" {
 namespace {
 
 // features rounded to bf16 on the host, exactly as the device rounds
@@ -899,6 +970,7 @@ bool all_finite(const float* x, size_t n) {
 }
 
 }  // namespace
+}  // extern "C++"
 
 int care_mlp_gradient(care_context* ctx, const float* features, const uint8_t* labels, int n, const float* params,
                       float* grad_out, float* loss_out) {
@@ -1112,6 +1184,116 @@ int care_gate_columns(care_context* ctx, uint64_t first, int n, float* features,
     } catch (const HipError& e) {
         return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
     }
+    return CARE_OK;
+}
+
+// Shape search: the query kernel over the LDS hit capacities that fit,
+// then the correction kernel over workgroup sizes and MSA staging chunks,
+// each timed twice (minimum kept) on the same sample of anchors.
+int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen) {
+    if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
+    if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
+    if (n <= 0 || first + uint64_t(n) > ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "range outside read store");
+    if (ctx->opt.paired_end && ((first & 1) || (n & 1)))
+        return fail(CARE_ERR_ARGUMENT, "paired-end samples must start at an even id and hold whole pairs");
+    const auto& o = ctx->opt;
+    CorrectParams P{};
+    if (const int rc = prepare_params(ctx, P)) return rc;
+    // no gate recording while timing
+    P.gateFeat = nullptr;
+    P.gateIds = nullptr;
+    P.gatePos = nullptr;
+    P.gateOrig = nullptr;
+    P.gateCons = nullptr;
+    P.gateCount = nullptr;
+    P.gateCap = 0;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int B = default_batch(ctx);
+        n = std::min(n, std::min(B, 65536)) & ~1;
+        if (n == 0) n = 2 > int(ctx->rs.n - first) ? 1 : 2;
+        ensure_batch(ctx, B);
+        hipStream_t s = ctx->stream;
+        const int candLimit = o.correction_type == 3 ? o.candidate_limit : 0;
+        auto timed = [&](auto&& launch) {
+            float best = 1e30f;
+            for (int rep = 0; rep < 2; rep++) {
+                HIP_TRY(hipEventRecord(ctx->ev[0], s));
+                launch();
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipEventRecord(ctx->ev[1], s));
+                HIP_TRY(hipEventSynchronize(ctx->ev[1]));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+                best = std::min(best, ms);
+            }
+            return best;
+        };
+        auto query = [&](int capL) {
+            launch_query(ctx->rs, ctx->tables, first, n, ctx->maxCand, capL, ctx->queryRegMax, o.exclude_ambiguous,
+                         candLimit,
+                         ctx->candIds.p, ctx->candCount.p, reinterpret_cast<u32*>(ctx->metaA.p), nullptr, s);
+        };
+        // query: LDS hit capacities up to the longest possible list
+        const int full = (ctx->maxCand + 15) / 16 * 16;
+        int bestCap = ctx->queryCap;
+        float bestQ = 1e30f;
+        for (int capL : {128, 256, 512, 1024, 1536, full}) {
+            if (capL > full || 4 * query_wave_lds_bytes(capL, candLimit) > size_t(kMaxLdsBytes)) continue;
+            const float ms = timed([&] { query(capL); });
+            if (ms < bestQ) {
+                bestQ = ms;
+                bestCap = capL;
+            }
+        }
+        ctx->queryCap = bestCap;
+        query(bestCap);
+        HIP_TRY(hipGetLastError());
+        // correction: 2 or 4 wavefronts per anchor, automatic and halved MSA chunks
+        P.candIds = ctx->candIds.p;
+        P.candCount = ctx->candCount.p;
+        P.metaA = ctx->metaA.p;
+        P.metaB = ctx->metaB.p;
+        P.outStatus = ctx->outStatus.p;
+        P.outSeq = ctx->outSeq.p;
+        P.outPitch = ctx->outPitch;
+        P.candOutIds = ctx->candOutIds.p;
+        P.candOutSeq = ctx->candOutSeq.p;
+        P.candOutCount = ctx->counters.p + 1;
+        P.candOutCap = ctx->candOutCap;
+        P.queueHead = ctx->counters.p;
+        P.firstAnchor = first;
+        P.numAnchors = n;
+        const int gatedK = o.correction_type == 3;
+        int bestT = ctx->blockThreads, bestChunk = ctx->msaChunk;
+        float bestC = 1e30f;
+        for (int T : {256, 128}) {
+            const int autoChunk = correct_msa_chunk(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T, gatedK, 0);
+            for (int chunk : {0, autoChunk / 2}) {
+                if (chunk != 0 && (chunk < 16 || chunk == autoChunk)) continue;
+                const size_t lds = correct_lds_bytes(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T, gatedK, chunk);
+                if (lds > size_t(kMaxLdsBytes)) continue;
+                const int bpc = ctx->blocksPerCU > 0 ? ctx->blocksPerCU : correct_resident_blocks(T, lds, gatedK);
+                const int blocks = std::max(1, std::min(n, ctx->numCUs * bpc));
+                const float ms = timed([&] {
+                    launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);
+                    launch_correct(P, blocks, T, chunk, s);
+                });
+                if (ms < bestC) {
+                    bestC = ms;
+                    bestT = T;
+                    bestChunk = chunk;
+                }
+            }
+        }
+        ctx->blockThreads = bestT;
+        ctx->msaChunk = bestChunk;
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    } catch (const std::bad_alloc&) {
+        return fail(CARE_ERR_DEVICE, "out of host memory");
+    }
+    if (chosen) return care_get_tuning(ctx, chosen);
     return CARE_OK;
 }
 

@@ -980,7 +980,8 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
 
 // LDS shape of k_correct; MSA chunks hold up to 256 candidates and about
 // 16 KiB of staged rows
-static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads, int gated) {
+// chunk: candidates staged per MSA pass (0 = about 16 KB of staged rows)
+static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads, int gated, int chunk) {
     LdsDims d;
     d.pitch = (maxLen + 15) / 16;
     d.qpitch = useQuality ? qpitch : 0;
@@ -989,7 +990,7 @@ static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads, in
     d.cols = 3 * maxLen + 2;
     d.waves = threads / kWave;
     const int rowBytes = int(sizeof(CandMeta)) + 4 * (d.pitch + d.qpitch);
-    d.chunk = std::max(16, std::min(256, 16384 / rowBytes));
+    d.chunk = chunk > 0 ? chunk : std::max(16, std::min(256, 16384 / rowBytes));
     d.gated = gated;
     if (gated) {
         // the staging area doubles as the gate's feature tile and MLP scratch
@@ -999,12 +1000,24 @@ static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads, in
     return d;
 }
 
-size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads, int gated) {
-    return make_dims(maxLen, useQuality, qpitch, threads, gated).bytes();
+size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads, int gated, int chunk) {
+    return make_dims(maxLen, useQuality, qpitch, threads, gated, chunk).bytes();
 }
 
-void launch_correct(const CorrectParams& p, int blocks, int threads, hipStream_t stream) {
-    const LdsDims d = make_dims(p.rs.maxLen, p.useQuality, p.rs.qpitch, threads, p.correctionType == 3);
+int correct_msa_chunk(int maxLen, int useQuality, int qpitch, int threads, int gated, int chunk) {
+    return make_dims(maxLen, useQuality, qpitch, threads, gated, chunk).chunk;
+}
+
+int correct_resident_blocks(int threads, size_t ldsBytes, int gated) {
+    int nb = 0;
+    const hipError_t e =
+        gated ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<true>, threads, ldsBytes)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<false>, threads, ldsBytes);
+    return e == hipSuccess && nb > 0 ? nb : 1;
+}
+
+void launch_correct(const CorrectParams& p, int blocks, int threads, int chunk, hipStream_t stream) {
+    const LdsDims d = make_dims(p.rs.maxLen, p.useQuality, p.rs.qpitch, threads, p.correctionType == 3, chunk);
     if (p.correctionType == 3)
         hipLaunchKernelGGL(k_correct<true>, dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
     else

@@ -14,6 +14,10 @@ using u32 = std::uint32_t;
 using u64 = std::uint64_t;
 
 constexpr u64 kEmptyKey = ~u64(0);
+// dynamic LDS a kernel of the engine may request per workgroup
+constexpr int kMaxLdsBytes = 64 * 1024;
+// default longest hit list a query wavefront merges in LDS
+constexpr int kDefaultQueryCap = 512;
 
 // Read store resident in HBM.  Rows are padded to `pitch` words (2-bit
 // sequences) and `qpitch` words (binned qualities).
@@ -149,13 +153,27 @@ struct CorrectParams {
 
 // launchers (all asynchronous on `stream`)
 void launch_signatures(const DevReadStore& rs, u64 first, int n, int k, int h, u64* out, hipStream_t stream);
-void launch_build_map(const DevReadStore& rs, const u64* sigs, int h, int map, int cap, u64* tkeys, u32* tcount,
-                      u64 tmask, u32* tfill, u64* scanTmp, u32* scanOut, hipStream_t stream);
-void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, int maxCand, int excludeAmbiguous,
-                  int candLimit, u32* candIds, int* candCount, unsigned long long* rawHits, hipStream_t stream);
-void launch_correct(const CorrectParams& p, int blocks, int threads, hipStream_t stream);
-size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads, int gated);
-int query_lds_bytes(int h, int cap, int maxLen, int threads, int candLimit);
+// LDS of one query wavefront: header (hit-list prefix counts and list
+// pointers), two merge buffers of capL ids, capL hit counts with candLimit
+__host__ __device__ inline size_t query_wave_lds_bytes(int capL, int candLimit) {
+    return (size_t(800) + 8 * size_t(capL) + (candLimit > 0 ? size_t(capL) : 0) + 15) / 16 * 16;
+}
+// spill: 4 * maxCand words per anchor of the batch (sort/merge buffers of
+// anchors with more than capL hits); regMax: longest hit list sorted in
+// registers (0 = the kernel's maximum, 1024), longer lists are merged;
+// rawCount (may be null): table hits per anchor before de-duplication
+void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, int maxCand, int capL, int regMax,
+                  int excludeAmbiguous, int candLimit, u32* candIds, int* candCount, u32* spill, int* rawCount,
+                  hipStream_t stream);
+// totals[0] += sum(rawCount[0..n)), totals[1] += sum(candCount[0..n))
+void launch_sum_counts(const int* rawCount, const int* candCount, int n, unsigned long long* totals, hipStream_t s);
+// chunk: candidates staged per MSA pass (0 = automatic)
+void launch_correct(const CorrectParams& p, int blocks, int threads, int chunk, hipStream_t stream);
+size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads, int gated, int chunk);
+int correct_msa_chunk(int maxLen, int useQuality, int qpitch, int threads, int gated, int chunk);
+// correction workgroups resident per compute unit at once (occupancy)
+int correct_resident_blocks(int threads, size_t ldsBytes, int gated);
+int query_lds_bytes(int h, int cap, int maxLen, int threads, int candLimit, int capL);
 
 // learned gate (mlp.hip)
 void launch_mlp_grad(const unsigned short* X, const u8* Y, const u32* perm, u64 first, int rows,

@@ -261,184 +261,400 @@ void launch_sort_buckets(const u32* toff, const u32* tcount, u64 tsize, u32* val
 }
 
 // ---------------------------------------------------------------------------
-// Query: one block per anchor.  Signature, probes of all maps, gather of
-// the hit lists into LDS, bitonic sort, unique, self/ambiguous removal.
+// Query: one wavefront per anchor.  Lane j probes map j with the anchor's
+// stored signature value j, the h hit lists (each sorted ascending) are
+// gathered and merged pairwise, then unique, self and ambiguous reads are
+// removed in order.  Hit lists of up to `capL` entries are merged in the
+// wave's LDS slice; longer ones (heavy repeats) in the anchor's rows of the
+// global scratch `spill` (4 * maxCand words per anchor).
 // ---------------------------------------------------------------------------
 // With candLimit > 0 only the candLimit candidates hit by the most maps are
 // kept (ties broken by smaller read id), still in ascending id order.
-__global__ void k_query(DevReadStore rs, DevTables t, u64 first, int n, int maxCand, int sortCap, int excludeAmbiguous,
-                        int candLimit, u32* candIds, int* candCount, unsigned long long* rawHits) {
-    HIP_DYNAMIC_SHARED(u64, lds64)
-    u64* canon = lds64;                                  // maxLen
-    u64* partial = canon + rs.maxLen;                    // blockDim
-    u64* sig = partial + blockDim.x;                     // h
-    int* counts = reinterpret_cast<int*>(sig + t.h);     // h + 1 (prefix)
-    int* scanScratch = counts + t.h + 2;                 // 17
-    u32* ids = reinterpret_cast<u32*>(scanScratch + 18); // sortCap
-    u8* hits = reinterpret_cast<u8*>(ids + sortCap);      // maxCand (candLimit > 0 only)
-    int* hist = counts;                                  // h + 1 bins, reused after the gather
-    __shared__ int sTotal, sThresh, sNeed;
+namespace {
 
-    for (int a = blockIdx.x; a < n; a += gridDim.x) {
+constexpr int kQueryHeader = 800;  // per wave: counts[h + 1] and list pointers[h]
+
+__device__ __forceinline__ void wave_sync_mem() {
+    // orders the wave's LDS and global writes before other lanes' reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ u64 lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+constexpr int kMergeIlp = 4;  // entries per lane searched together (independent LDS chains)
+
+// gathers the hit lists into a and merges them; returns the buffer (a or b)
+// that holds the sorted entries.  Round r merges adjacent runs of 2^r lists
+// (left run [l0, mid), right run [mid, r1)): an entry's merged position is
+// its rank in its own run plus the entries of the other run before it
+// (left: strictly smaller, right: smaller or equal), a stable merge.  Both
+// searches (the entry's merged run, its rank in the other run) are
+// fixed-step binary liftings over kMergeIlp entries at a time, so a lane
+// has kMergeIlp independent LDS reads in flight instead of one chain.
+template <class Buf>
+__device__ __forceinline__ Buf gather_merge(Buf a, Buf b, const int* counts, const u32* const* lists, int h,
+                                            int total, int lane) {
+    for (int i = lane; i < total; i += kWave) {
+        int lo = 0, hi = h - 1;  // last list j with counts[j] <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (counts[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        a[i] = lists[lo][i - counts[lo]];
+    }
+    wave_sync_mem();
+    Buf src = a, dst = b;
+    for (int r = 0; (1 << r) < h; r++) {
+        const int span = 2 << r;  // lists per merged run
+        const int nseg = (h + span - 1) / span;
+        // longest merged run of this round (bounds the rank search)
+        int longest = lane < nseg ? counts[min(lane * span + span, h)] - counts[lane * span] : 0;
+        for (int o = 32; o > 0; o >>= 1) longest = max(longest, __shfl_xor(longest, o, kWave));
+        int segTop = 1, runTop = 1;
+        while (2 * segTop <= nseg - 1) segTop *= 2;
+        while (2 * runTop <= longest) runTop *= 2;
+        for (int base = 0; base < total; base += kMergeIlp * kWave) {
+            u32 x[kMergeIlp];
+            int sg[kMergeIlp], rank[kMergeIlp], ob[kMergeIlp], olen[kMergeIlp], start[kMergeIlp], l0[kMergeIlp];
+            bool left[kMergeIlp];
+#pragma unroll
+            for (int e = 0; e < kMergeIlp; e++) {
+                const int i = base + e * kWave + lane;
+                x[e] = i < total ? src[i] : 0u;
+                sg[e] = 0;
+            }
+            // last merged run starting at or before i
+            for (int step = segTop; step > 0; step >>= 1) {
+#pragma unroll
+                for (int e = 0; e < kMergeIlp; e++) {
+                    const int i = base + e * kWave + lane;
+                    const int t = sg[e] + step;
+                    if (t < nseg && counts[t * span] <= i) sg[e] = t;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < kMergeIlp; e++) {
+                const int i = base + e * kWave + lane;
+                const int p0 = sg[e] * span;
+                l0[e] = counts[p0];
+                const int mid = counts[min(p0 + (span >> 1), h)];
+                const int r1 = counts[min(p0 + span, h)];
+                left[e] = i < mid;
+                ob[e] = left[e] ? mid : l0[e];
+                olen[e] = left[e] ? r1 - mid : mid - l0[e];
+                start[e] = left[e] ? l0[e] : mid;
+                rank[e] = 0;
+            }
+            // entries of the other run before x: a prefix of that run
+            for (int step = runTop; step > 0; step >>= 1) {
+#pragma unroll
+                for (int e = 0; e < kMergeIlp; e++) {
+                    const int t = rank[e] + step;
+                    if (t <= olen[e]) {
+                        const u32 v = src[ob[e] + t - 1];
+                        if (left[e] ? v < x[e] : v <= x[e]) rank[e] = t;
+                    }
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < kMergeIlp; e++) {
+                const int i = base + e * kWave + lane;
+                if (i < total) dst[l0[e] + (i - start[e]) + rank[e]] = x[e];
+            }
+        }
+        wave_sync_mem();
+        const Buf t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+// Bitonic sort of 64*K entries held in registers, entry e = lane*K + r in
+// register r: strides below K stay inside a lane, larger ones exchange with
+// lane ^ (stride / K).  Every loop bound is a constant, so the network is
+// fully unrolled and v stays in registers.
+template <int K>
+__device__ __forceinline__ void sort_in_registers(u32 (&v)[K], int lane) {
+#pragma unroll
+    for (int size = 2; size <= kWave * K; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= K) {
+                const int lx = stride / K;
+#pragma unroll
+                for (int r = 0; r < K; r++) {
+                    const u32 o = __shfl_xor(v[r], lx, kWave);
+                    const int e = lane * K + r;
+                    const bool up = (e & size) == 0;
+                    const bool lower = (e & stride) == 0;
+                    v[r] = (lower == up) ? min(v[r], o) : max(v[r], o);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < K; r++) {
+                    if (r & stride) continue;
+                    const int e = lane * K + r;
+                    const bool up = (e & size) == 0;
+                    const u32 a = v[r], b = v[r + stride];
+                    v[r] = up ? min(a, b) : max(a, b);
+                    v[r + stride] = up ? max(a, b) : min(a, b);
+                }
+            }
+        }
+    }
+}
+
+// gathers the hit lists (total <= 64*K) straight into registers, padded
+// with ~0u (sorts last), sorts them and stores the sorted entries to out
+template <int K, class Buf>
+__device__ __forceinline__ void gather_sort(Buf out, const int* counts, const u32* const* lists, int h, int total,
+                                            int lane) {
+    u32 v[K];
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const int e = lane * K + r;
+        u32 x = 0xFFFFFFFFu;
+        if (e < total) {
+            int lo = 0, hi = h - 1;  // last list j with counts[j] <= e
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (counts[mid] <= e) lo = mid;
+                else hi = mid - 1;
+            }
+            x = lists[lo][e - counts[lo]];
+        }
+        v[r] = x;
+    }
+    sort_in_registers<K>(v, lane);
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const int e = lane * K + r;
+        if (e < total) out[e] = v[r];
+    }
+    wave_sync_mem();
+}
+
+// longest hit list sorted in registers (16 per lane: longer networks cost
+// more registers than they save); longer lists are merged
+constexpr int kRegisterSortMax = 16 * kWave;
+
+template <class Buf>
+__device__ __forceinline__ void gather_sort_any(Buf out, const int* counts, const u32* const* lists, int h, int total,
+                                                int lane) {
+    if (total <= 2 * kWave) gather_sort<2>(out, counts, lists, h, total, lane);
+    else if (total <= 4 * kWave) gather_sort<4>(out, counts, lists, h, total, lane);
+    else if (total <= 8 * kWave) gather_sort<8>(out, counts, lists, h, total, lane);
+    else gather_sort<16>(out, counts, lists, h, total, lane);
+}
+
+// unique, self/ambiguous removal and compaction into out; hit counts
+// (lengths of the runs of equal ids) into hits when candLimit > 0
+template <class Buf, class HitBuf>
+__device__ __forceinline__ int compact_unique(Buf sorted, int total, u32 self, const DevReadStore& rs,
+                                              int excludeAmbiguous, int candLimit, u32* out, HitBuf hits, int lane) {
+    int written = 0;
+    const u64 below = lanes_below(lane);
+    for (int base = 0; base < total; base += kWave) {
+        const int i = base + lane;
+        bool keep = false;
+        u32 v = 0;
+        if (i < total) {
+            v = sorted[i];
+            keep = (i == 0 || sorted[i - 1] != v) && v != self;
+            if (keep && excludeAmbiguous) keep = rs.ambig[v] == 0;
+        }
+        const u64 bal = __ballot(keep);
+        const int pos = written + __popcll(bal & below);
+        if (keep) {
+            out[pos] = v;
+            if (candLimit > 0) {
+                int run = 1;  // maps that hit v
+                while (i + run < total && sorted[i + run] == v) run++;
+                hits[pos] = u8(run);
+            }
+        }
+        written += __popcll(bal);
+    }
+    return written;
+}
+
+// keeps the candLimit candidates with the most hits (ties: smaller id, i.e.
+// earlier in out), in place and in order; hist holds h + 1 ints
+template <class HitBuf>
+__device__ __forceinline__ int keep_top(u32* out, HitBuf hits, int written, int candLimit, int h, int* hist,
+                                        int lane) {
+    for (int b = lane; b <= h; b += kWave) hist[b] = 0;
+    wave_sync_mem();
+    for (int i = lane; i < written; i += kWave) atomicAdd(&hist[hits[i]], 1);
+    wave_sync_mem();
+    int T = 0, need = 0;
+    if (lane == 0) {
+        int above = 0, v = h;
+        while (v > 0 && above + hist[v] < candLimit) above += hist[v--];
+        T = v;
+        need = candLimit - above;
+    }
+    T = __shfl(T, 0, kWave);
+    need = __shfl(need, 0, kWave);
+    const u64 below = lanes_below(lane);
+    int kept = 0, atT = 0;
+    for (int base = 0; base < written; base += kWave) {
+        const int i = base + lane;
+        u32 v = 0;
+        int hc = 0;
+        if (i < written) {
+            v = out[i];
+            hc = hits[i];
+        }
+        const u64 bT = __ballot(i < written && hc == T);
+        const int rT = __popcll(bT & below);
+        const bool keep = i < written && (hc > T || (hc == T && atT + rT < need));
+        const u64 bk = __ballot(keep);
+        if (keep) out[kept + __popcll(bk & below)] = v;  // kept + rank <= i: reads of later entries unaffected
+        kept += __popcll(bk);
+        atT += __popcll(bT);
+    }
+    return kept;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_query(DevReadStore rs, DevTables t, u64 first, int n, int maxCand, int capL,
+                                               int regMax, int excludeAmbiguous, int candLimit, u32* candIds,
+                                               int* candCount, u32* spill, int* rawCount) {
+    HIP_DYNAMIC_SHARED(uint4, ldsRaw)
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    const int wpb = blockDim.x / kWave;
+    const int h = t.h;
+    char* wl = reinterpret_cast<char*>(ldsRaw) + size_t(wave) * query_wave_lds_bytes(capL, candLimit);
+    int* counts = reinterpret_cast<int*>(wl);                       // h + 1, reused as hit histogram
+    const u32** lists = reinterpret_cast<const u32**>(wl + 288);     // h
+    u32* la = reinterpret_cast<u32*>(wl + kQueryHeader);            // capL
+    u32* lb = la + capL;                                             // capL
+    u8* lh = reinterpret_cast<u8*>(lb + capL);                       // capL (candLimit > 0)
+
+    for (int a = blockIdx.x * wpb + wave; a < n; a += gridDim.x * wpb) {
         const u64 id = first + u64(a);
         const int len = rs.lens[id];
         const bool skip = (excludeAmbiguous && rs.ambig[id]) || len < t.k;
         if (skip) {
-            if (threadIdx.x == 0) candCount[a] = 0;
-            continue;  // uniform per block
+            if (lane == 0) {
+                candCount[a] = 0;
+                if (rawCount) rawCount[a] = 0;
+            }
+            continue;  // uniform per wave
         }
-        if (t.sigs) {
-            // signatures computed once for the table build
-            for (int j = threadIdx.x; j < t.h; j += blockDim.x) sig[j] = t.sigs[id * u64(t.h) + j];
-            __syncthreads();
-        } else {
-            block_signature(rs.seqs + id * rs.pitch, len, t.k, t.h, canon, partial, sig);
-        }
-        // probe every map
-        for (int j = threadIdx.x; j < t.h; j += blockDim.x) {
-            const DevMap m = t.maps[j];
-            const u64 slot = find_slot(m.keys, m.mask, sig[j]);
-            int c = 0;
+        // lane j probes map j with the stored signature value j
+        int c = 0;
+        const u32* list = nullptr;
+        if (lane < h) {
+            const DevMap m = t.maps[lane];
+            const u64 slot = find_slot(m.keys, m.mask, t.sigs[id * u64(h) + lane]);
             if (slot != ~u64(0)) {
                 const uint2 oc = m.offcnt[slot];
                 c = int(oc.y);
-                partial[j] = (u64(oc.x) << 32) | u64(oc.y);  // reuse partial as (offset,count)
-            } else {
-                partial[j] = 0;
-            }
-            counts[j] = c;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int run = 0;
-            for (int j = 0; j < t.h; j++) {
-                const int c = counts[j];
-                counts[j] = run;
-                run += c;
-            }
-            counts[t.h] = run;
-            sTotal = run;
-        }
-        __syncthreads();
-        const int total = sTotal;
-        if (rawHits && threadIdx.x == 0) atomicAdd(rawHits, (unsigned long long)total);
-        // gather (total <= h * cap <= sortCap by construction)
-        const int n2 = int(next_pow2(unsigned(total > 0 ? total : 1)));
-        for (int i = threadIdx.x; i < n2; i += blockDim.x) {
-            u32 v = 0xFFFFFFFFu;
-            if (i < total) {
-                int lo = 0, hi = t.h - 1;  // last j with counts[j] <= i
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (counts[mid] <= i) lo = mid;
-                    else hi = mid - 1;
-                }
-                const u64 oc = partial[lo];
-                const u32 off = u32(oc >> 32);
-                v = t.maps[lo].values[u64(off) + u64(i - counts[lo])];
-            }
-            ids[i] = v;
-        }
-        __syncthreads();
-        // bitonic sort of n2 entries
-        for (int size = 2; size <= n2; size <<= 1) {
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int i = threadIdx.x; i < n2 / 2; i += blockDim.x) {
-                    const int lo = 2 * i - (i & (stride - 1));
-                    const int hi = lo + stride;
-                    const bool up = (lo & size) == 0;
-                    const u32 x = ids[lo], y = ids[hi];
-                    if ((x > y) == up) {
-                        ids[lo] = y;
-                        ids[hi] = x;
-                    }
-                }
-                __syncthreads();
+                list = m.values + oc.x;
             }
         }
-        // unique, drop self and (optionally) ambiguous reads, compact in order
+        int incl = c;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int v = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += v;
+        }
+        const int total = __shfl(incl, kWave - 1, kWave);
+        if (lane < h) {
+            counts[lane] = incl - c;
+            lists[lane] = list;
+        }
+        if (lane == 0) counts[h] = total;
+        wave_sync_mem();
+
         u32* out = candIds + u64(a) * maxCand;
-        int written = 0;
-        for (int base = 0; base < total; base += blockDim.x) {
-            const int i = base + threadIdx.x;
-            bool keep = false;
-            u32 v = 0;
-            if (i < total) {
-                v = ids[i];
-                keep = (i == 0 || ids[i - 1] != v) && v != u32(id);
-                if (keep && excludeAmbiguous) keep = rs.ambig[v] == 0;
-            }
-            int cnt;
-            const int pos = block_exclusive_count(keep, scanScratch, &cnt);
-            if (keep && written + pos < maxCand) {
-                out[written + pos] = v;
-                if (candLimit > 0) {
-                    int run = 1;  // maps that hit v: length of its run in the sorted hits
-                    while (i + run < total && ids[i + run] == v) run++;
-                    hits[written + pos] = u8(run);
-                }
-            }
-            written += cnt;
+        u32* gs = spill + u64(a) * 4 * maxCand;  // [maxCand] + [maxCand] + hit bytes
+        u8* gh = reinterpret_cast<u8*>(gs + 2 * maxCand);
+        int written;
+        if (total <= capL) {
+            const u32* sorted = la;
+            if (total <= regMax) gather_sort_any(la, counts, lists, h, total, lane);
+            else sorted = gather_merge(la, lb, counts, lists, h, total, lane);
+            written = compact_unique(sorted, total, u32(id), rs, excludeAmbiguous, candLimit, out, lh, lane);
+        } else {
+            const u32* sorted = gs;
+            if (total <= regMax) gather_sort_any(gs, counts, lists, h, total, lane);
+            else sorted = gather_merge(gs, gs + maxCand, counts, lists, h, total, lane);
+            written = compact_unique(sorted, total, u32(id), rs, excludeAmbiguous, candLimit, out, gh, lane);
         }
-        if (written > maxCand) written = maxCand;
+        wave_sync_mem();
         if (candLimit > 0 && written > candLimit) {
-            // threshold hit count T: keep every candidate above T and the
-            // first (in id order) candidates at T up to candLimit
-            __syncthreads();
-            for (int b = threadIdx.x; b <= t.h; b += blockDim.x) hist[b] = 0;
-            __syncthreads();
-            for (int i = threadIdx.x; i < written; i += blockDim.x) atomicAdd(&hist[hits[i]], 1);
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                int above = 0, v = t.h;
-                while (v > 0 && above + hist[v] < candLimit) above += hist[v--];
-                sThresh = v;
-                sNeed = candLimit - above;
-            }
-            __syncthreads();
-            const int T = sThresh, need = sNeed;
-            int kept = 0, atT = 0;
-            for (int base = 0; base < written; base += blockDim.x) {
-                const int i = base + threadIdx.x;
-                u32 v = 0;
-                int hc = 0;
-                if (i < written) {
-                    v = out[i];
-                    hc = hits[i];
-                }
-                int nT;
-                const int rT = block_exclusive_count(i < written && hc == T, scanScratch, &nT);
-                const bool keep = i < written && (hc > T || (hc == T && atT + rT < need));
-                int cnt;
-                const int pos = block_exclusive_count(keep, scanScratch, &cnt);
-                if (keep) out[kept + pos] = v;
-                kept += cnt;
-                atT += nT;
-            }
-            written = kept;
+            written = total <= capL ? keep_top(out, lh, written, candLimit, h, counts, lane)
+                                    : keep_top(out, gh, written, candLimit, h, counts, lane);
         }
-        if (threadIdx.x == 0) {
+        if (lane == 0) {
             candCount[a] = written;
-            if (rawHits) atomicAdd(rawHits + 1, (unsigned long long)written);
+            if (rawCount) rawCount[a] = total;
         }
-        __syncthreads();
+        wave_sync_mem();  // counts and buffers are rewritten by the next anchor
     }
 }
 
-int query_lds_bytes(int h, int cap, int maxLen, int threads, int candLimit) {
-    const int sortCap = int(1u << (32 - __builtin_clz(unsigned(h * cap > 1 ? h * cap - 1 : 1))));
-    return int(sizeof(u64) * (size_t(maxLen) + threads + h) + sizeof(int) * (h + 2 + 18) + sizeof(u32) * sortCap +
-               (candLimit > 0 ? size_t(h) * cap : 0));
+// totals[0] += sum of raw hit counts, totals[1] += sum of candidate counts
+// (one workgroup, so no atomics)
+__global__ void __launch_bounds__(1024) k_sum_counts(const int* rawCount, const int* candCount, int n,
+                                                     unsigned long long* totals) {
+    __shared__ unsigned long long part[2][16];
+    unsigned long long r = 0, c = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        r += u64(rawCount[i]);
+        c += u64(candCount[i]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        r += __shfl_xor(r, o, kWave);
+        c += __shfl_xor(c, o, kWave);
+    }
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        part[0][w] = r;
+        part[1][w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < int(blockDim.x / kWave); k++) {
+            r += part[0][k];
+            c += part[1][k];
+        }
+        totals[0] += r;
+        totals[1] += c;
+    }
 }
 
-void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, int maxCand, int excludeAmbiguous,
-                  int candLimit, u32* candIds, int* candCount, unsigned long long* rawHits, hipStream_t stream) {
+int query_lds_bytes(int h, int cap, int maxLen, int threads, int candLimit, int capL) {
+    (void)h;
+    (void)cap;
+    (void)maxLen;
+    return int(size_t(threads / kWave) * query_wave_lds_bytes(capL, candLimit));
+}
+
+void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, int maxCand, int capL, int regMax,
+                  int excludeAmbiguous, int candLimit, u32* candIds, int* candCount, u32* spill, int* rawCount,
+                  hipStream_t stream) {
     if (n <= 0) return;
     const int threads = 256;
-    const int lds = query_lds_bytes(t.h, maxCand / t.h, rs.maxLen, threads, candLimit);
-    const int sortCap = int(1u << (32 - __builtin_clz(unsigned(maxCand > 1 ? maxCand - 1 : 1))));
-    hipLaunchKernelGGL(k_query, dim3(n), dim3(threads), lds, stream, rs, t, first, n, maxCand, sortCap,
-                       excludeAmbiguous, candLimit, candIds, candCount, rawHits);
+    const int wpb = threads / kWave;
+    const int lds = query_lds_bytes(t.h, maxCand / t.h, rs.maxLen, threads, candLimit, capL);
+    const int blocks = std::min((n + wpb - 1) / wpb, 1 << 16);
+    regMax = regMax > 0 ? std::min(regMax, kRegisterSortMax) : kRegisterSortMax;
+    hipLaunchKernelGGL(k_query, dim3(blocks), dim3(threads), lds, stream, rs, t, first, n, maxCand, capL, regMax,
+                       excludeAmbiguous, candLimit, candIds, candCount, spill, rawCount);
+}
+
+void launch_sum_counts(const int* rawCount, const int* candCount, int n, unsigned long long* totals, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, s, rawCount, candCount, n, totals);
 }
 
 }  // namespace care_engine

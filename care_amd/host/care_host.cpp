@@ -308,6 +308,7 @@ struct HostOptions {
     bool showProgress = false;
     std::vector<int> gpus;
     int batchsize = 0;
+    bool autotune = true;  // time launch shapes on a sample of anchors before STEP 2
     std::string forestAnchor, forestCands;
     std::string gateFile;  // learned gate parameters (correctionType 3)
     uint32_t maxTreesAnchor = 0xFFFFFFFFu, maxTreesCands = 0xFFFFFFFFu;
@@ -391,6 +392,7 @@ HostOptions parseOptions(int argc, char** argv) {
             if (v > 0) o.maxTreesCands = uint32_t(v);
         }
         else if (a == "--batchsize") o.batchsize = std::stoi(value());
+        else if (a == "--autotune") o.autotune = flag();  // --autotune=false keeps the default shapes
         else if (a == "--gzoutput") o.gzoutput = flag();
         else if (a == "-p" || a == "--showProgress") o.showProgress = flag();
         else if (a == "--enforceHashmapCount") (void)flag();
@@ -592,6 +594,13 @@ int runMain(int argc, char** argv) {
     // STEP 2
     std::printf("STEP 2: Error correction\n");
     const u64 numAnchors = o.fixedNumberOfReads ? std::min<u64>(o.fixedNumberOfReads, rs->n) : rs->n;
+    if (o.autotune && numAnchors >= 4096) {
+        // launch shapes timed on the first anchors; results do not depend on them
+        care_tuning t{};
+        check(care_autotune(ctx, 0, int(std::min<u64>(numAnchors, 65536) & ~u64(1)), &t));
+        std::printf("Launch shapes: query %d hits in LDS, correction %d threads x %d per CU, MSA chunk %d\n",
+                    t.query_lds_hits, t.correct_threads, t.correct_blocks_per_cu, t.msa_chunk);
+    }
     const int pitch = (rs->maxLen + 15) / 16 * 16;
     const int chunk = 1 << 18;
     std::vector<u8> status(chunk);

@@ -134,9 +134,11 @@ void addRepeats(Rng& rng, std::string& genome, int families, int minLen, int max
 // U[1,5] kbp with 5-20 copies are added until the fraction is reached.
 void addRepeatFraction(Rng& rng, std::string& genome, double fraction, double divergence, double gc) {
     const double target = fraction * double(genome.size());
+    if (genome.size() <= 1000) return;  // no copy of 1..5 kbp fits
     double covered = 0;
     while (covered < target) {
         const int len = 1000 + int(rng.below(4001));
+        if (std::size_t(len) >= genome.size()) continue;  // copies shorter than the genome
         const int copies = 5 + int(rng.below(16));
         std::string tmpl(len, 'A');
         for (auto& c : tmpl) c = randomBase(rng, gc);

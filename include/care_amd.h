@@ -125,9 +125,44 @@ typedef struct care_batch_result {
  * candidatecorrection,correction}kernels.cu). */
 int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* result);
 
-/* Number of anchors per internal launch and launch shape chosen by the
- * autotuner (0 = automatic). */
+/* Number of anchors per internal launch (even, 0 = automatic) and threads
+ * per correction workgroup (128 or 256, 0 = keep); the other launch shapes
+ * stay as set by care_set_tuning / care_autotune.  The reference fixes these
+ * at compile time (e.g. blocksize 128 in src/gpu/alignmentkernels.cu:652,
+ * src/gpu/msakernels.cu:1186). */
 int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threads);
+
+/* Launch shapes of the engine's kernels.  None of them changes a result:
+ * every shape gives bit-identical candidate lists and corrections.  No
+ * counterpart in the reference, which fixes its launch shapes at compile
+ * time (src/gpu/alignmentkernels.cu:652, src/gpu/msakernels.cu:1186). */
+typedef struct care_tuning {
+    int batch_anchors;          /* anchors per internal launch, even (0 = automatic) */
+    int correct_threads;        /* threads per correction workgroup, i.e. 2 or 4 wavefronts
+                                   per anchor: 128 or 256 (0 = 256) */
+    int correct_blocks_per_cu;  /* persistent correction workgroups per compute unit
+                                   (0 = as many as are resident at once) */
+    int msa_chunk;              /* candidates staged in LDS per MSA pass, 16..256
+                                   (0 = automatic: about 16 KB of rows) */
+    int query_lds_hits;         /* longest hit list a query wavefront sorts in LDS,
+                                   multiple of 16 in 16..4096; longer lists use global
+                                   scratch (0 = 512) */
+    int query_register_hits;    /* longest hit list sorted in registers (bitonic network
+                                   across the wavefront), power of two in 64..1024;
+                                   longer lists are merged list by list (0 = 1024) */
+} care_tuning;
+
+int care_set_tuning(care_context* ctx, const care_tuning* tuning);
+/* Current shapes with automatic values resolved. */
+int care_get_tuning(care_context* ctx, care_tuning* out);
+
+/* Times candidate shapes of the query and correction kernels on anchors
+ * [first, first+n) of the built tables (n even for paired-end, first even;
+ * at most 65536 anchors are used), keeps the fastest as the context's
+ * tuning and writes it to `chosen` (may be NULL).  Costs a few query and
+ * correction launches over the sample; results of later calls do not
+ * depend on the outcome. */
+int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen);
 
 /* Forest classifier for correction_type / correction_type_cands 1, in the
  * reference's .rf file format (include/forest.hpp:38-55, ml/care.py:198-282).

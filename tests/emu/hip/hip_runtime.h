@@ -477,6 +477,13 @@ inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
     *v = emu::num_cus();
     return hipSuccess;
 }
+// blocks resident per compute unit: bounded by 64 KB of LDS and 8 blocks
+template <class K>
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* nb, K, int, size_t lds) {
+    const size_t byLds = (64 * 1024) / (lds ? lds : 1);
+    *nb = int(byLds < 8 ? (byLds ? byLds : 1) : 8);
+    return hipSuccess;
+}
 template <class T>
 inline hipError_t hipMalloc(T** p, size_t bytes) {
     *p = static_cast<T*>(std::calloc(1, bytes ? bytes : 1));
