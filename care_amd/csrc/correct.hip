@@ -255,12 +255,15 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
 }
 
 // Alignment of the anchor against the candidate planes of one wave, all
-// shifts in parallel; result key = score << 11 | order index, or ~0u.
-// For shift s >= 0 the anchor is read from base s against the candidate
-// from base 0, for s < 0 the candidate from base -s against the anchor from
-// base 0: one side is always word aligned.  The word loop is uniform over
-// the wave and ends when no lane is still below its mismatch limit; reads
-// past a lane's last word stay inside the workgroup's LDS and are masked.
+// shifts in parallel, both orientations of a shift in the same lane; result
+// key = score << 11 | order index, or ~0u.  For shift s >= 0 the anchor is
+// read from base s against the candidate from base 0, for s < 0 the
+// candidate from base -s against the anchor from base 0: one side is always
+// word aligned, and the anchor side is shared by the two orientations.  The
+// word loop is uniform over the wave and ends when no lane is still below
+// its mismatch limit in either orientation (mismatch counts only grow, so
+// counting past the limit changes no result); reads past a lane's last word
+// stay inside the workgroup's LDS and are masked.
 __device__ __forceinline__ void align_wave(const uint2* ap, int la, const uint2* fp, const uint2* rp, int lc,
                                            int minov, float maxErrorRate, u32* keyF, u32* keyR) {
     const int lane = lane_id();
@@ -269,37 +272,41 @@ __device__ __forceinline__ void align_wave(const uint2* ap, int la, const uint2*
     const int tot = nPos + nNeg;
     const int total = la + lc;
     u32 kf = ~0u, kr = ~0u;
-    for (int t0 = 0; t0 < 2 * tot; t0 += kWave) {
-        const int t = t0 + lane;
-        const bool inRange = t < 2 * tot;
-        const bool isRc = t >= tot;
-        const int u = inRange ? (isRc ? t - tot : t) : 0;
+    for (int u0 = 0; u0 < tot; u0 += kWave) {
+        const int u = u0 + lane;
+        const bool inRange = u < tot;
         const int s = u < nPos ? u : -(u - nPos + 1);
         const int ov = s >= 0 ? min(la - s, lc) : min(la, lc + s);
         const int lim = int(float(ov) * maxErrorRate);
         const int nw = (inRange && lim > 0) ? (ov + 31) >> 5 : 0;
-        const uint2* cp = isRc ? rp : fp;
         const int d = s >= 0 ? s : -s;
         const u32 sh = u32(32 - (d & 31)) & 31u;
-        const uint2* up = (s >= 0 ? ap : cp) + (d >> 5) + ((d & 31) != 0);
-        const uint2* vp = (s >= 0 ? cp : ap) + 1;
+        const int off = (d >> 5) + ((d & 31) != 0);
+        const uint2* p0 = (s >= 0 ? ap : fp) + off;  // shifted side, forward candidate
+        const uint2* p1 = (s >= 0 ? ap : rp) + off;  // shifted side, reverse-complement candidate
+        const uint2* q0 = (s >= 0 ? fp : ap) + 1;    // word-aligned side
+        const uint2* q1 = (s >= 0 ? rp : ap) + 1;
         const u32 lastMask = (ov & 31) ? ~(~0u >> (ov & 31)) : ~0u;
-        uint2 w0 = up[0];
-        int mm = 0;
-        for (int j = 0; __any(j < nw && mm < lim); j++) {
-            const uint2 w1 = up[j + 1];
-            const uint2 v = vp[j];
-            const u32 xh = __builtin_amdgcn_alignbit(w0.x, w1.x, sh) ^ v.x;
-            const u32 xl = __builtin_amdgcn_alignbit(w0.y, w1.y, sh) ^ v.y;
+        uint2 a0 = p0[0], b0 = p1[0];
+        int mf = 0, mr = 0;
+        for (int j = 0; __any(j < nw && (mf < lim || mr < lim)); j++) {
+            const uint2 a1 = p0[j + 1], b1 = p1[j + 1];
+            const uint2 va = q0[j], vb = q1[j];
             u32 msk = j + 1 < nw ? ~0u : lastMask;
             msk = j < nw ? msk : 0u;
-            mm += __popc((xh | xl) & msk);
-            w0 = w1;
+            const u32 xf = (__builtin_amdgcn_alignbit(a0.x, a1.x, sh) ^ va.x) |
+                           (__builtin_amdgcn_alignbit(a0.y, a1.y, sh) ^ va.y);
+            const u32 xr = (__builtin_amdgcn_alignbit(b0.x, b1.x, sh) ^ vb.x) |
+                           (__builtin_amdgcn_alignbit(b0.y, b1.y, sh) ^ vb.y);
+            mf += __popc(xf & msk);
+            mr += __popc(xr & msk);
+            a0 = a1;
+            b0 = b1;
         }
-        if (nw > 0 && mm < lim) {
-            const u32 key = (u32(mm + total - 2 * ov) << 11) | u32(u);
-            if (isRc) kr = key < kr ? key : kr;
-            else kf = key < kf ? key : kf;
+        if (nw > 0) {
+            const u32 base = u32(total - 2 * ov);
+            if (mf < lim) kf = min(kf, ((u32(mf) + base) << 11) | u32(u));
+            if (mr < lim) kr = min(kr, ((u32(mr) + base) << 11) | u32(u));
         }
     }
     *keyF = wave_min_u32(kf);
@@ -579,87 +586,103 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         }
         CandMeta* metaA = P.metaA + u64(a) * P.maxCand;
         CandMeta* metaB = P.metaB + u64(a) * P.maxCand;
-        for (int c = wave; c < n; c += nwaves) {
-            const u32 cid = ids[c];
-            const int lc = rs.lens[cid];
-            const u32* cenc = rs.seqs + u64(cid) * rs.pitch;
-            const int cw = (lc + 31) / 32;
-            for (int p = lane; p < pw; p += kWave) {
-                const int w = p - 1;
-                u32 hi = 0, lo = 0;
-                if (w >= 0 && w < cw) {
-                    const u32 x0 = cenc[2 * w];
-                    const u32 x1 = 2 * w + 1 < rs.pitch ? cenc[2 * w + 1] : 0u;
-                    hi = (even_bits(x0 >> 1) << 16) | even_bits(x1 >> 1);
-                    lo = (even_bits(x0) << 16) | even_bits(x1);
-                    const int valid = lc - 32 * w;
-                    if (valid < 32) {
-                        const u32 mask = ~0u << (32 - valid);
-                        hi &= mask;
-                        lo &= mask;
-                    }
+        // candidate ids, lengths and 2-bit rows of a chunk are gathered into
+        // LDS by the whole block, so their global loads are in flight together
+        for (int c0 = 0; c0 < n; c0 += dims.chunk) {
+            const int nc = min(dims.chunk, n - c0);
+            __syncthreads();  // the previous chunk's rows are no longer read
+            for (int i = tid; i < nc * rs.pitch; i += blockDim.x) {
+                const int c = i / rs.pitch, w = i - c * rs.pitch;
+                const u32 cid = ids[c0 + c];
+                L.rows[i] = rs.seqs[u64(cid) * rs.pitch + w];
+                if (w == 0) {
+                    L.mst[c].id = cid;
+                    L.mst[c].len = u16(rs.lens[cid]);
                 }
-                fp[p] = make_uint2(hi, lo);
             }
-            wave_sync();
-            // reverse complement planes: rc[q] = 1 - f[lc-1-q]
-            for (int p = lane; p < pw; p += kWave) {
-                const int w = p - 1;
-                u32 hi = 0, lo = 0;
-                if (w >= 0 && w < cw) {
-                    const int e = lc - 32 * (w + 1);  // first fwd bit of the window
-                    u32 xh, xl;
-                    if (e >= 0) {
-                        const int k = e >> 5, r = e & 31;
-                        const uint2 a0 = fp[1 + k], a1 = fp[2 + k];
-                        xh = r ? (a0.x << r) | (a1.x >> (32 - r)) : a0.x;
-                        xl = r ? (a0.y << r) | (a1.y >> (32 - r)) : a0.y;
-                    } else {
-                        xh = fp[1].x >> (-e);
-                        xl = fp[1].y >> (-e);
+            __syncthreads();
+            for (int c = wave; c < nc; c += nwaves) {
+                const u32 cid = L.mst[c].id;
+                const int lc = L.mst[c].len;
+                const u32* cenc = L.rows + c * rs.pitch;
+                const int cw = (lc + 31) / 32;
+                for (int p = lane; p < pw; p += kWave) {
+                    const int w = p - 1;
+                    u32 hi = 0, lo = 0;
+                    if (w >= 0 && w < cw) {
+                        const u32 x0 = cenc[2 * w];
+                        const u32 x1 = 2 * w + 1 < rs.pitch ? cenc[2 * w + 1] : 0u;
+                        hi = (even_bits(x0 >> 1) << 16) | even_bits(x1 >> 1);
+                        lo = (even_bits(x0) << 16) | even_bits(x1);
+                        const int valid = lc - 32 * w;
+                        if (valid < 32) {
+                            const u32 mask = ~0u << (32 - valid);
+                            hi &= mask;
+                            lo &= mask;
+                        }
                     }
-                    hi = ~__brev(xh);
-                    lo = ~__brev(xl);
-                    const int valid = lc - 32 * w;
-                    if (valid < 32) {
-                        const u32 mask = ~0u << (32 - valid);
-                        hi &= mask;
-                        lo &= mask;
-                    }
+                    fp[p] = make_uint2(hi, lo);
                 }
-                rp[p] = make_uint2(hi, lo);
-            }
-            wave_sync();
-            u32 kf, kr;
-            align_wave(L.ap, la, fp, rp, lc, minovA, P.maxErrorRate, &kf, &kr);
-            if (lane == 0) {
-                const Aln f = decode_key(kf, la, lc, minovA);
-                const Aln r = decode_key(kr, la, lc, minovA);
-                const int flag = choose_orientation(f, r, la, P.minOverlapRatio, P.minOverlap, P.maxErr4);
-                const Aln& ch = flag == 2 ? r : f;
-                CandMeta m;
-                m.id = cid;
-                m.shift = short(ch.shift);
-                m.len = u16(lc);
-                m.u.al.nOps = u16(ch.nOps);
-                m.u.al.overlap = u16(ch.overlap);
-                m.flag = u8(flag);
-                u8 paired = 0;
-                if (mateIds) {
-                    const u32 want = cid ^ 1u;
-                    int lo2 = 0, hi2 = mateN;
-                    while (lo2 < hi2) {
-                        const int mid = (lo2 + hi2) >> 1;
-                        if (mateIds[mid] < want) lo2 = mid + 1;
-                        else hi2 = mid;
+                wave_sync();
+                // reverse complement planes: rc[q] = 1 - f[lc-1-q]
+                for (int p = lane; p < pw; p += kWave) {
+                    const int w = p - 1;
+                    u32 hi = 0, lo = 0;
+                    if (w >= 0 && w < cw) {
+                        const int e = lc - 32 * (w + 1);  // first fwd bit of the window
+                        u32 xh, xl;
+                        if (e >= 0) {
+                            const int k = e >> 5, r = e & 31;
+                            const uint2 a0 = fp[1 + k], a1 = fp[2 + k];
+                            xh = r ? (a0.x << r) | (a1.x >> (32 - r)) : a0.x;
+                            xl = r ? (a0.y << r) | (a1.y >> (32 - r)) : a0.y;
+                        } else {
+                            xh = fp[1].x >> (-e);
+                            xl = fp[1].y >> (-e);
+                        }
+                        hi = ~__brev(xh);
+                        lo = ~__brev(xl);
+                        const int valid = lc - 32 * w;
+                        if (valid < 32) {
+                            const u32 mask = ~0u << (32 - valid);
+                            hi &= mask;
+                            lo &= mask;
+                        }
                     }
-                    paired = (lo2 < mateN && mateIds[lo2] == want) ? 1 : 0;
+                    rp[p] = make_uint2(hi, lo);
                 }
-                m.paired = paired;
-                m.pad = 0;
-                metaA[c] = m;
+                wave_sync();
+                u32 kf, kr;
+                align_wave(L.ap, la, fp, rp, lc, minovA, P.maxErrorRate, &kf, &kr);
+                if (lane == 0) {
+                    const Aln f = decode_key(kf, la, lc, minovA);
+                    const Aln r = decode_key(kr, la, lc, minovA);
+                    const int flag = choose_orientation(f, r, la, P.minOverlapRatio, P.minOverlap, P.maxErr4);
+                    const Aln& ch = flag == 2 ? r : f;
+                    CandMeta m;
+                    m.id = cid;
+                    m.shift = short(ch.shift);
+                    m.len = u16(lc);
+                    m.u.al.nOps = u16(ch.nOps);
+                    m.u.al.overlap = u16(ch.overlap);
+                    m.flag = u8(flag);
+                    u8 paired = 0;
+                    if (mateIds) {
+                        const u32 want = cid ^ 1u;
+                        int lo2 = 0, hi2 = mateN;
+                        while (lo2 < hi2) {
+                            const int mid = (lo2 + hi2) >> 1;
+                            if (mateIds[mid] < want) lo2 = mid + 1;
+                            else hi2 = mid;
+                        }
+                        paired = (lo2 < mateN && mateIds[lo2] == want) ? 1 : 0;
+                    }
+                    m.paired = paired;
+                    m.pad = 0;
+                    metaA[c0 + c] = m;
+                }
+                wave_sync();
             }
-            wave_sync();
         }
         __syncthreads();
 
