@@ -139,9 +139,13 @@ struct Msa {
 // (src/msa.cpp:9-173).  One thread per column; per column the weights are
 // summed anchor first, then candidates in index order, as on the CPU.
 // Candidate records, 2-bit rows and quality rows are staged through LDS in
-// chunks of dims.chunk candidates.  All threads must call.
+// chunks of dims.chunk candidates.  Only the columns within `margin` of the
+// anchor are computed: every reader of the MSA (refinement, properties,
+// anchor correction and its features, candidate correction within
+// new_columns of the anchor) stays inside them, and columns are independent
+// of each other.  All threads must call.
 __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& dims, const CandMeta* meta, int m,
-                         int la, int* sMin, int* sMax) {
+                         int la, int margin, int* sMin, int* sMax) {
     if (threadIdx.x == 0) {
         *sMin = 0;
         *sMax = la;
@@ -169,9 +173,10 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
     const int r0S = int(threadIdx.x) / pitch, wS = int(threadIdx.x) - r0S * pitch;
     const int rowsQ = qpitch ? int(blockDim.x) / qpitch : 0;
     const int r0Q = qpitch ? int(threadIdx.x) / qpitch : 0, wQ = int(threadIdx.x) - r0Q * qpitch;
-    for (int cb = 0; cb < cols; cb += blockDim.x) {
+    const int wb = max(0, msa.ab - margin), we = min(cols, ae + margin);
+    for (int cb = wb; cb < we; cb += blockDim.x) {
         const int col = cb + int(threadIdx.x);
-        const bool active = col < cols;
+        const bool active = col < we;
         int cA = 0, cC = 0, cG = 0, cT = 0;
         float wA = 0.f, wC = 0.f, wG = 0.f, wT = 0.f;
         // conditional adds of +0.0f leave the other sums bit-identical
@@ -749,7 +754,10 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         // ---- phase C: MSA + refinement (include/corrector.hpp:1307-1421) ----
         CandMeta* cur = metaB;
         CandMeta* other = metaA;
-        Msa msa = build_msa(P, L, dims, cur, m, la, &sMin, &sMax);
+        // MSA columns read beyond the anchor: neighbours of gate features (2),
+        // candidate correction (new_columns)
+        const int margin = P.correctCandidates ? max(2, P.newColumns) : 2;
+        Msa msa = build_msa(P, L, dims, cur, m, la, margin, &sMin, &sMax);
         const int maxIter = kGated ? 0 : 5;  // the gated mode skips refinement
         for (int iter = 0; iter < maxIter && m > 0; iter++) {
             // src/msa.cpp:532-734
@@ -823,7 +831,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             cur = other;
             other = t;
             m = m2;
-            msa = build_msa(P, L, dims, cur, m, la, &sMin, &sMax);
+            msa = build_msa(P, L, dims, cur, m, la, margin, &sMin, &sMax);
         }
 
         // ---- phase D: anchor correction (src/msa.cpp:304-453, include/corrector.hpp:1423-1472) ----
