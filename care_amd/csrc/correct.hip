@@ -591,8 +591,9 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         }
         CandMeta* metaA = P.metaA + u64(a) * P.maxCand;
         CandMeta* metaB = P.metaB + u64(a) * P.maxCand;
-        // candidate ids, lengths and 2-bit rows of a chunk are gathered into
-        // LDS by the whole block, so their global loads are in flight together
+        // candidate ids, lengths, 2-bit rows and pair flags of a chunk are
+        // gathered into LDS by the whole block, so their global loads (and
+        // the binary searches in the mate's candidate list) run in parallel
         for (int c0 = 0; c0 < n; c0 += dims.chunk) {
             const int nc = min(dims.chunk, n - c0);
             __syncthreads();  // the previous chunk's rows are no longer read
@@ -603,6 +604,20 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 if (w == 0) {
                     L.mst[c].id = cid;
                     L.mst[c].len = u16(rs.lens[cid]);
+                    // the candidate's mate is a candidate of the anchor's mate
+                    // (include/corrector.hpp:763-807)
+                    u8 paired = 0;
+                    if (mateIds) {
+                        const u32 want = cid ^ 1u;
+                        int lo2 = 0, hi2 = mateN;
+                        while (lo2 < hi2) {
+                            const int mid = (lo2 + hi2) >> 1;
+                            if (mateIds[mid] < want) lo2 = mid + 1;
+                            else hi2 = mid;
+                        }
+                        paired = (lo2 < mateN && mateIds[lo2] == want) ? 1 : 0;
+                    }
+                    L.mst[c].paired = paired;
                 }
             }
             __syncthreads();
@@ -671,18 +686,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     m.u.al.nOps = u16(ch.nOps);
                     m.u.al.overlap = u16(ch.overlap);
                     m.flag = u8(flag);
-                    u8 paired = 0;
-                    if (mateIds) {
-                        const u32 want = cid ^ 1u;
-                        int lo2 = 0, hi2 = mateN;
-                        while (lo2 < hi2) {
-                            const int mid = (lo2 + hi2) >> 1;
-                            if (mateIds[mid] < want) lo2 = mid + 1;
-                            else hi2 = mid;
-                        }
-                        paired = (lo2 < mateN && mateIds[lo2] == want) ? 1 : 0;
-                    }
-                    m.paired = paired;
+                    m.paired = L.mst[c].paired;
                     m.pad = 0;
                     metaA[c0 + c] = m;
                 }
