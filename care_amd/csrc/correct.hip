@@ -66,7 +66,7 @@ struct Lds {
     u32* aw;        // anchor 2-bit words [pitch]
     uint2* ap;      // anchor planes [hw + 3]
     char* anchorQ;  // [maxLen]
-    uint2* planes;  // per wave: forward and reverse-complement planes, 2 x [hw + 3]
+    uint2* cpl;     // [chunk] candidate planes: forward, then reverse complement, 2 x [hw + 3]
     u8* cons;       // [cols]
     float* support;
     int* coverage;
@@ -89,7 +89,7 @@ struct LdsDims {
         b += sizeof(u32) * pitch;
         b += sizeof(uint2) * pw();
         b += (size_t(maxLen) + 3) / 4 * 4;
-        b += size_t(waves) * 2 * sizeof(uint2) * pw();
+        b += size_t(chunk) * 2 * sizeof(uint2) * pw();
         b += (size_t(cols) + 3) / 4 * 4;
         b += sizeof(float) * cols + 2 * sizeof(int) * cols;
         b += 4 * sizeof(u16) * cols + 4 * sizeof(float) * cols;
@@ -105,7 +105,7 @@ __device__ Lds carve(char* base, const LdsDims& d) {
     // 16-byte aligned base: 16- and 8-byte records first
     l.mst = reinterpret_cast<CandMeta*>(p); p += sizeof(CandMeta) * d.chunk;
     l.ap = reinterpret_cast<uint2*>(p); p += sizeof(uint2) * d.pw();
-    l.planes = reinterpret_cast<uint2*>(p); p += size_t(d.waves) * 2 * sizeof(uint2) * d.pw();
+    l.cpl = reinterpret_cast<uint2*>(p); p += size_t(d.chunk) * 2 * sizeof(uint2) * d.pw();
     l.rows = reinterpret_cast<u32*>(p); p += sizeof(u32) * size_t(d.chunk) * d.pitch;
     l.qrows = reinterpret_cast<u32*>(p); p += sizeof(u32) * size_t(d.chunk) * d.qpitch;
     l.qw = reinterpret_cast<float*>(p); p += sizeof(float) * 256;
@@ -294,19 +294,26 @@ __device__ __forceinline__ void align_wave(const uint2* ap, int la, const uint2*
         const u32 lastMask = (ov & 31) ? ~(~0u >> (ov & 31)) : ~0u;
         uint2 a0 = p0[0], b0 = p1[0];
         int mf = 0, mr = 0;
-        for (int j = 0; __any(j < nw && (mf < lim || mr < lim)); j++) {
-            const uint2 a1 = p0[j + 1], b1 = p1[j + 1];
-            const uint2 va = q0[j], vb = q1[j];
+        // two words per iteration: one LDS round trip covers 64 bases
+        for (int j = 0; __any(j < nw && (mf < lim || mr < lim)); j += 2) {
+            const uint2 a1 = p0[j + 1], b1 = p1[j + 1], a2 = p0[j + 2], b2 = p1[j + 2];
+            const uint2 va = q0[j], vb = q1[j], va2 = q0[j + 1], vb2 = q1[j + 1];
             u32 msk = j + 1 < nw ? ~0u : lastMask;
             msk = j < nw ? msk : 0u;
+            u32 msk2 = j + 2 < nw ? ~0u : lastMask;
+            msk2 = j + 1 < nw ? msk2 : 0u;
             const u32 xf = (__builtin_amdgcn_alignbit(a0.x, a1.x, sh) ^ va.x) |
                            (__builtin_amdgcn_alignbit(a0.y, a1.y, sh) ^ va.y);
             const u32 xr = (__builtin_amdgcn_alignbit(b0.x, b1.x, sh) ^ vb.x) |
                            (__builtin_amdgcn_alignbit(b0.y, b1.y, sh) ^ vb.y);
-            mf += __popc(xf & msk);
-            mr += __popc(xr & msk);
-            a0 = a1;
-            b0 = b1;
+            const u32 xf2 = (__builtin_amdgcn_alignbit(a1.x, a2.x, sh) ^ va2.x) |
+                            (__builtin_amdgcn_alignbit(a1.y, a2.y, sh) ^ va2.y);
+            const u32 xr2 = (__builtin_amdgcn_alignbit(b1.x, b2.x, sh) ^ vb2.x) |
+                            (__builtin_amdgcn_alignbit(b1.y, b2.y, sh) ^ vb2.y);
+            mf += __popc(xf & msk) + __popc(xf2 & msk2);
+            mr += __popc(xr & msk) + __popc(xr2 & msk2);
+            a0 = a2;
+            b0 = b2;
         }
         if (nw > 0) {
             const u32 base = u32(total - 2 * ov);
@@ -524,8 +531,6 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
     const int wave = tid / kWave;
     const int nwaves = blockDim.x / kWave;
     const int pw = dims.pw();
-    uint2* fp = L.planes + size_t(wave) * 2 * pw;
-    uint2* rp = fp + pw;
 
     if (P.useQuality) {
         for (int i = tid; i < 256; i += blockDim.x) L.qw[i] = P.qweights[i];
@@ -621,57 +626,64 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 }
             }
             __syncthreads();
+            // bit planes of every staged candidate: forward ...
+            for (int it = tid; it < nc * pw; it += blockDim.x) {
+                const int c = it / pw, p = it - c * pw;
+                const int lc = L.mst[c].len;
+                const u32* cenc = L.rows + c * rs.pitch;
+                const int w = p - 1;  // plane word (index 0 is padding)
+                u32 hi = 0, lo = 0;
+                if (w >= 0 && w < (lc + 31) / 32) {
+                    const u32 x0 = cenc[2 * w];
+                    const u32 x1 = 2 * w + 1 < rs.pitch ? cenc[2 * w + 1] : 0u;
+                    hi = (even_bits(x0 >> 1) << 16) | even_bits(x1 >> 1);
+                    lo = (even_bits(x0) << 16) | even_bits(x1);
+                    const int valid = lc - 32 * w;
+                    if (valid < 32) {
+                        const u32 mask = ~0u << (32 - valid);
+                        hi &= mask;
+                        lo &= mask;
+                    }
+                }
+                L.cpl[size_t(2 * c) * pw + p] = make_uint2(hi, lo);
+            }
+            __syncthreads();
+            // ... and reverse complement: rc[q] = 1 - f[lc-1-q]
+            for (int it = tid; it < nc * pw; it += blockDim.x) {
+                const int c = it / pw, p = it - c * pw;
+                const int lc = L.mst[c].len;
+                const uint2* fp = L.cpl + size_t(2 * c) * pw;
+                const int w = p - 1;
+                u32 hi = 0, lo = 0;
+                if (w >= 0 && w < (lc + 31) / 32) {
+                    const int e = lc - 32 * (w + 1);  // first fwd bit of the window
+                    u32 xh, xl;
+                    if (e >= 0) {
+                        const int k = e >> 5, r = e & 31;
+                        const uint2 a0 = fp[1 + k], a1 = fp[2 + k];
+                        xh = r ? (a0.x << r) | (a1.x >> (32 - r)) : a0.x;
+                        xl = r ? (a0.y << r) | (a1.y >> (32 - r)) : a0.y;
+                    } else {
+                        xh = fp[1].x >> (-e);
+                        xl = fp[1].y >> (-e);
+                    }
+                    hi = ~__brev(xh);
+                    lo = ~__brev(xl);
+                    const int valid = lc - 32 * w;
+                    if (valid < 32) {
+                        const u32 mask = ~0u << (32 - valid);
+                        hi &= mask;
+                        lo &= mask;
+                    }
+                }
+                L.cpl[size_t(2 * c + 1) * pw + p] = make_uint2(hi, lo);
+            }
+            __syncthreads();
             for (int c = wave; c < nc; c += nwaves) {
                 const u32 cid = L.mst[c].id;
                 const int lc = L.mst[c].len;
-                const u32* cenc = L.rows + c * rs.pitch;
-                const int cw = (lc + 31) / 32;
-                for (int p = lane; p < pw; p += kWave) {
-                    const int w = p - 1;
-                    u32 hi = 0, lo = 0;
-                    if (w >= 0 && w < cw) {
-                        const u32 x0 = cenc[2 * w];
-                        const u32 x1 = 2 * w + 1 < rs.pitch ? cenc[2 * w + 1] : 0u;
-                        hi = (even_bits(x0 >> 1) << 16) | even_bits(x1 >> 1);
-                        lo = (even_bits(x0) << 16) | even_bits(x1);
-                        const int valid = lc - 32 * w;
-                        if (valid < 32) {
-                            const u32 mask = ~0u << (32 - valid);
-                            hi &= mask;
-                            lo &= mask;
-                        }
-                    }
-                    fp[p] = make_uint2(hi, lo);
-                }
-                wave_sync();
-                // reverse complement planes: rc[q] = 1 - f[lc-1-q]
-                for (int p = lane; p < pw; p += kWave) {
-                    const int w = p - 1;
-                    u32 hi = 0, lo = 0;
-                    if (w >= 0 && w < cw) {
-                        const int e = lc - 32 * (w + 1);  // first fwd bit of the window
-                        u32 xh, xl;
-                        if (e >= 0) {
-                            const int k = e >> 5, r = e & 31;
-                            const uint2 a0 = fp[1 + k], a1 = fp[2 + k];
-                            xh = r ? (a0.x << r) | (a1.x >> (32 - r)) : a0.x;
-                            xl = r ? (a0.y << r) | (a1.y >> (32 - r)) : a0.y;
-                        } else {
-                            xh = fp[1].x >> (-e);
-                            xl = fp[1].y >> (-e);
-                        }
-                        hi = ~__brev(xh);
-                        lo = ~__brev(xl);
-                        const int valid = lc - 32 * w;
-                        if (valid < 32) {
-                            const u32 mask = ~0u << (32 - valid);
-                            hi &= mask;
-                            lo &= mask;
-                        }
-                    }
-                    rp[p] = make_uint2(hi, lo);
-                }
-                wave_sync();
+                const uint2* fp = L.cpl + size_t(2 * c) * pw;
+                const uint2* rp = fp + pw;
                 u32 kf, kr;
                 align_wave(L.ap, la, fp, rp, lc, minovA, P.maxErrorRate, &kf, &kr);
                 if (lane == 0) {
