@@ -472,8 +472,13 @@ __device__ __forceinline__ int compact_unique(Buf sorted, int total, u32 self, c
         if (keep) {
             out[pos] = v;
             if (candLimit > 0) {
-                int run = 1;  // maps that hit v
-                while (i + run < total && sorted[i + run] == v) run++;
+                // maps that hit v: length of its run of equal entries (at most
+                // h <= 64), found by binary lifting
+                int run = 1;
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int t = run + step;
+                    if (t <= 64 && i + t - 1 < total && sorted[i + t - 1] == v) run = t;
+                }
                 hits[pos] = u8(run);
             }
         }
@@ -491,15 +496,25 @@ __device__ __forceinline__ int keep_top(u32* out, HitBuf hits, int written, int 
     wave_sync_mem();
     for (int i = lane; i < written; i += kWave) atomicAdd(&hist[hits[i]], 1);
     wave_sync_mem();
-    int T = 0, need = 0;
-    if (lane == 0) {
-        int above = 0, v = h;
-        while (v > 0 && above + hist[v] < candLimit) above += hist[v--];
-        T = v;
-        need = candLimit - above;
+    // threshold T: the largest hit count at which the candidates with at
+    // least T hits reach candLimit (0 if they never do); lane l holds the
+    // count of hit number h - l, prefix sums run from h downwards
+    const int cnt = lane < h ? hist[h - lane] : 0;
+    int incl = cnt;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int v = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += v;
     }
-    T = __shfl(T, 0, kWave);
-    need = __shfl(need, 0, kWave);
+    const u64 reach = __ballot(lane < h && incl >= candLimit);
+    int T, need;
+    if (reach) {
+        const int l = __ffsll((long long)reach) - 1;
+        T = h - l;
+        need = candLimit - (__shfl(incl, l, kWave) - __shfl(cnt, l, kWave));
+    } else {
+        T = 0;
+        need = candLimit - __shfl(incl, h - 1, kWave);
+    }
     const u64 below = lanes_below(lane);
     int kept = 0, atT = 0;
     for (int base = 0; base < written; base += kWave) {

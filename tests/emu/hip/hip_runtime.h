@@ -412,6 +412,7 @@ inline float __uint_as_float(unsigned x) {
 }
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz(unsigned(x)); }
 inline unsigned __brev(unsigned x) {
     unsigned r = 0;
