@@ -1036,7 +1036,8 @@ static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads, in
     d.maxLen = maxLen;
     d.cols = 3 * maxLen + 2;
     d.waves = threads / kWave;
-    const int rowBytes = int(sizeof(CandMeta)) + 4 * (d.pitch + d.qpitch);
+    // LDS per staged candidate: record, 2-bit and quality rows, bit planes
+    const int rowBytes = int(sizeof(CandMeta)) + 4 * (d.pitch + d.qpitch) + 2 * int(sizeof(uint2)) * d.pw();
     d.chunk = chunk > 0 ? chunk : std::max(16, std::min(256, 16384 / rowBytes));
     d.gated = gated;
     if (gated) {
