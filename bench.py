@@ -51,6 +51,17 @@ WORKLOADS = {
         "max_length": 100,
         "label": "uniform-se: 1 Mbp genome, 100 bp SE at 30x (300k reads), k=20, h=16",
     },
+    "large-pe": {
+        "profile": "large-pe", "seed": 20240605, "genome": 100000000,
+        "args": ["-c", "40", "--pairmode", "PE", "-q", "-k", "20", "-h", "48", "--candidateCorrection"],
+        "options": dict(coverage=40.0, paired_end=1, use_quality=1, kmer_length=20, num_hash_maps=48,
+                        correct_candidates=1),
+        "qual_bits": 8,
+        "max_length": 150,
+        "truth": False,  # 8.5 GB of reads; base-level quality is measured on the smaller workloads
+        "label": "large-pe: 100 Mbp genome, 5% repeats at 2% divergence, 2x150 bp PE at 40x (26.7M reads), "
+                 "-q 8-bit, k=20, h=48, --candidateCorrection",
+    },
 }
 
 HBM_PEAK_GBS = 8000.0
@@ -65,11 +76,13 @@ def prepare(workload, cache):
     w = WORKLOADS[workload]
     os.makedirs(cache, exist_ok=True)
     reads = os.path.join(cache, "%s_%d_%d.fastq" % (w["profile"], w["seed"], w["genome"]))
-    truth = reads + ".truth"
+    truth = reads + ".truth" if w.get("truth", True) else os.devnull
     if not os.path.exists(reads):
         log("bench: simulating %s" % w["label"])
+        t0 = time.perf_counter()
         care_amd.simulate(w["profile"], w["seed"], reads + ".tmp", truth, genome=w["genome"])
         os.replace(reads + ".tmp", reads)
+        log("bench: simulated in %.1f s" % (time.perf_counter() - t0))
     return reads, truth
 
 
@@ -120,18 +133,42 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=Tru
     # result arrays allocated once and reused by every step, as a caller
     # correcting batch after batch would
     pitch = (rs.max_length + 15) // 16 * 16
-    out = (np.zeros(n, dtype=np.uint8), np.zeros((n, pitch), dtype=np.uint8))
-    out[1].fill(0)
+    # with candidate correction the reads go through in chunks of 2^18
+    # anchors (as the command line does), each with room for 64 candidate
+    # results per anchor; otherwise in one call
+    chunk = 1 << 18 if opt.correct_candidates else n
+    cap = chunk * 64 if opt.correct_candidates else 0
+    # result arrays in page-locked host memory (copied by DMA)
+    out = (care_amd.pinned_empty((chunk,), np.uint8), care_amd.pinned_empty((chunk, pitch), np.uint8),
+           care_amd.pinned_empty((max(cap, 1),), np.uint32), care_amd.pinned_empty((max(cap, 1), pitch), np.uint8))
+
+    class Totals:
+        pass
+
+    def step():
+        t = Totals()
+        t.query_ms = t.correct_ms = 0.0
+        t.launches = t.candidates = t.raw_hits = t.corrected = 0
+        for first in range(0, n, chunk):
+            nb = min(chunk, n - first)
+            status, _, _, _ = eng.correct(first, nb, candidate_capacity=cap, out=out)
+            s = eng.stats()
+            t.query_ms += s.query_ms
+            t.correct_ms += s.correct_ms
+            t.launches += s.launches
+            t.candidates += s.candidates
+            t.raw_hits += s.raw_hits
+            t.corrected += int((status[:nb] & 1).sum())
+        return t
+
     for i in range(warmup):
-        eng.correct(0, n, out=out)
+        step()
     step_ms, corr_ms, query_ms = [], [], []
     st = None
-    status = None
     for i in range(steps):
         ts = time.perf_counter()
-        status, _, _, _ = eng.correct(0, n, out=out)
+        st = step()
         te = time.perf_counter()
-        st = eng.stats()
         step_ms.append((te - ts) * 1e3)
         corr_ms.append(st.correct_ms)
         query_ms.append(st.query_ms)
@@ -141,7 +178,7 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=Tru
     return {"ms": sum(step_ms) / len(step_ms), "correct_ms": sum(corr_ms) / len(corr_ms),
             "query_ms": sum(query_ms) / len(query_ms), "launches": max(1, st.launches), "candidates": int(st.candidates),
             "raw_hits": int(st.raw_hits), "h": int(opt.num_hash_maps), "tuning": tuning,
-            "corrected": int((status & 1).sum())}
+            "corrected": int(st.corrected)}
 
 
 def quality(care_amd, reads, truth, w, extra, outdir):
@@ -200,7 +237,7 @@ def main():
     rs.close()
 
     qual = None
-    if not args.no_quality and os.path.exists(truth):
+    if not args.no_quality and w.get("truth", True) and os.path.exists(truth):
         work = os.path.join(os.path.dirname(reads), "quality")
         qual = {"plain": quality(care_amd, reads, truth, w, [], os.path.join(work, "plain"))}
         if gated_ok:

@@ -129,6 +129,8 @@ ENGINE_SYMBOLS = {
     "care_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(CareTuning)]),
     "care_forest_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint32]),
     "care_last_batch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchStats)]),
+    "care_host_buffer_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "care_host_buffer_free": (ctypes.c_int, [ctypes.c_void_p]),
     "care_mlp_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "care_mlp_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -191,6 +193,24 @@ def host():
 
 class CareError(RuntimeError):
     pass
+
+
+def pinned_empty(shape, dtype, lib=None):
+    """numpy array over page-locked host memory (care_host_buffer_alloc),
+    freed when the array is collected; result copies into it need no
+    staging."""
+    import weakref
+    lib = lib if lib is not None else engine()
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape)) if shape else 1
+    nbytes = max(1, count * dt.itemsize)
+    p = ctypes.c_void_p()
+    if lib.care_host_buffer_alloc(nbytes, ctypes.byref(p)) != CARE_OK:
+        raise CareError(lib.care_last_error().decode())
+    buf = (ctypes.c_char * nbytes).from_address(p.value)
+    arr = np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
+    weakref.finalize(buf, lib.care_host_buffer_free, ctypes.c_void_p(p.value))
+    return arr
 
 
 def _check(rc):
@@ -345,18 +365,24 @@ class Engine:
             return ids[: offs[n]], offs
 
     def correct(self, first, n, candidate_capacity=None, out=None):
-        """Corrects anchors [first, first+n).  out: (status, seqs) arrays to
-        reuse across calls (uint8 [n] and uint8 [n][pitch])."""
+        """Corrects anchors [first, first+n).  out: arrays reused across
+        calls, (status, seqs) or (status, seqs, cand_ids, cand_seqs):
+        uint8 [n], uint8 [n][pitch], uint32 [cap], uint8 [cap][pitch]."""
         pitch = (self.max_length + 15) // 16 * 16
+        ccap = candidate_capacity if candidate_capacity is not None else (n * 64 if self.options.correct_candidates else 0)
         if out is not None:
-            status, seqs = out
+            status, seqs = out[0], out[1]
             assert status.shape[0] >= n and seqs.shape[0] >= n and seqs.shape[1] == pitch
         else:
             status = np.zeros(n, dtype=np.uint8)
             seqs = np.zeros((n, pitch), dtype=np.uint8)
-        ccap = candidate_capacity if candidate_capacity is not None else (n * 64 if self.options.correct_candidates else 0)
-        cids = np.zeros(max(ccap, 1), dtype=np.uint32)
-        cseqs = np.zeros((max(ccap, 1), pitch), dtype=np.uint8)
+        if out is not None and len(out) == 4:
+            cids, cseqs = out[2], out[3]
+            ccap = min(ccap, cids.shape[0], cseqs.shape[0])
+            assert cseqs.shape[1] == pitch
+        else:
+            cids = np.zeros(max(ccap, 1), dtype=np.uint32)
+            cseqs = np.zeros((max(ccap, 1), pitch), dtype=np.uint8)
         r = BatchResult()
         r.anchor_status = status.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         r.anchor_sequences = seqs.ctypes.data_as(ctypes.POINTER(ctypes.c_char))

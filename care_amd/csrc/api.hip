@@ -710,24 +710,34 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
         P.candOutCount = ctx->counters.p + 1;
         P.candOutCap = ctx->candOutCap;
         P.queueHead = ctx->counters.p;
-        // Without candidate correction every device batch writes its own
-        // slice of call-wide output buffers, so batches run back to back with
-        // no host synchronisation and the results come back in two copies.
-        // With candidate correction the variable-size candidate output is
-        // collected after every batch.
-        const bool async = !o.correct_candidates;
+        // Every device batch writes its slice of call-wide anchor outputs and
+        // appends its candidate corrections to a call-wide candidate buffer
+        // (one counter over the call), so the batches run back to back with
+        // no host synchronisation and the results come back in a few copies
+        // at the end.  Candidate results come in no particular order: the
+        // per-read combination of results does not depend on it.
         const int nBatches = (n + B - 1) / B;
-        if (async && n > ctx->outCap) {
+        if (n > ctx->outCap) {
             ctx->outStatus.alloc(size_t(n));
             ctx->outSeq.alloc(size_t(n) * ctx->outPitch);
             ctx->outCap = n;
         }
+        const int candCap = o.correct_candidates ? std::max(0, r->candidate_capacity) : 0;
+        if (candCap > ctx->candOutCap) {
+            ctx->candOutIds.alloc(size_t(candCap));
+            ctx->candOutSeq.alloc(size_t(candCap) * ctx->outPitch);
+            ctx->candOutCap = candCap;
+        }
+        P.candOutIds = ctx->candOutIds.p;
+        P.candOutSeq = ctx->candOutSeq.p;
+        P.candOutCap = candCap;  // results beyond are dropped and reported below
         while (int(ctx->batchEv.size()) < 3 * nBatches) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
             ctx->batchEv.push_back(e);
         }
         launch_fill_u64(reinterpret_cast<u64*>(ctx->rawHits.p), 2, 0, s);  // totals over the call
+        launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);  // candidate results over the call
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
         int bi = 0;
         for (int done = 0; done < n; done += B, bi++) {
@@ -740,40 +750,32 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
                          reinterpret_cast<u32*>(ctx->metaA.p), ctx->rawCount.p, s);
             launch_sum_counts(ctx->rawCount.p, ctx->candCount.p, nb, ctx->rawHits.p, s);
             HIP_TRY(hipEventRecord(e[1], s));
-            launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);
+            launch_reset_counters(ctx->counters.p, nullptr, s);  // work queue of this batch
             P.firstAnchor = first + done;
             P.numAnchors = nb;
-            P.outStatus = ctx->outStatus.p + (async ? size_t(done) : 0);
-            P.outSeq = ctx->outSeq.p + (async ? size_t(done) * ctx->outPitch : 0);
+            P.outStatus = ctx->outStatus.p + size_t(done);
+            P.outSeq = ctx->outSeq.p + size_t(done) * ctx->outPitch;
             const int blocks = std::max(1, std::min(nb, ctx->numCUs * blocksPerCU));
             launch_correct(P, blocks, T, ctx->msaChunk, s);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipEventRecord(e[2], s));
             ctx->stats.launches++;
-            if (async) continue;
-            HIP_TRY(hipMemcpyAsync(r->anchor_status + done, ctx->outStatus.p, size_t(nb), hipMemcpyDeviceToHost, s));
-            copy_rows_to_host(r->anchor_sequences + size_t(done) * r->sequence_pitch, size_t(r->sequence_pitch),
-                              ctx->outSeq.p, size_t(ctx->outPitch), size_t(ctx->rs.maxLen), size_t(nb), s);
+        }
+        HIP_TRY(hipMemcpyAsync(r->anchor_status, ctx->outStatus.p, size_t(n), hipMemcpyDeviceToHost, s));
+        copy_rows_to_host(r->anchor_sequences, size_t(r->sequence_pitch), ctx->outSeq.p, size_t(ctx->outPitch),
+                          size_t(ctx->rs.maxLen), size_t(n), s);
+        if (o.correct_candidates) {
             HIP_TRY(hipMemcpyAsync(ctx->hInts.p, ctx->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             const int nc = ctx->hInts.p[1];
-            if (nc > ctx->candOutCap) return fail(CARE_ERR_CAPACITY, "candidate correction buffer overflow (%d)", nc);
-            if (r->num_candidates + nc > r->candidate_capacity)
-                return fail(CARE_ERR_CAPACITY, "candidate_capacity exceeded");
+            if (nc > candCap) return fail(CARE_ERR_CAPACITY, "candidate_capacity exceeded (%d results)", nc);
             if (nc > 0) {
-                HIP_TRY(hipMemcpyAsync(r->candidate_read_ids + r->num_candidates, ctx->candOutIds.p, sizeof(u32) * nc,
+                HIP_TRY(hipMemcpyAsync(r->candidate_read_ids, ctx->candOutIds.p, sizeof(u32) * size_t(nc),
                                        hipMemcpyDeviceToHost, s));
-                copy_rows_to_host(r->candidate_sequences + size_t(r->num_candidates) * r->sequence_pitch,
-                                  size_t(r->sequence_pitch), ctx->candOutSeq.p, size_t(ctx->outPitch),
-                                  size_t(ctx->rs.maxLen), size_t(nc), s);
-                HIP_TRY(hipStreamSynchronize(s));
+                copy_rows_to_host(r->candidate_sequences, size_t(r->sequence_pitch), ctx->candOutSeq.p,
+                                  size_t(ctx->outPitch), size_t(ctx->rs.maxLen), size_t(nc), s);
             }
-            r->num_candidates += nc;
-        }
-        if (async) {
-            HIP_TRY(hipMemcpyAsync(r->anchor_status, ctx->outStatus.p, size_t(n), hipMemcpyDeviceToHost, s));
-            copy_rows_to_host(r->anchor_sequences, size_t(r->sequence_pitch), ctx->outSeq.p, size_t(ctx->outPitch),
-                              size_t(ctx->rs.maxLen), size_t(n), s);
+            r->num_candidates = nc;
         }
         HIP_TRY(hipMemcpyAsync(ctx->hU64.p, ctx->rawHits.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipEventRecord(ctx->ev[3], s));
@@ -806,6 +808,21 @@ extern "C" {
 int care_last_batch_stats(care_context* ctx, care_batch_stats* stats) {
     if (!ctx || !stats) return fail(CARE_ERR_ARGUMENT, "null argument");
     *stats = ctx->stats;
+    return CARE_OK;
+}
+
+int care_host_buffer_alloc(size_t bytes, void** out) {
+    if (!out) return fail(CARE_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    const hipError_t e = hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(CARE_ERR_DEVICE, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return CARE_OK;
+}
+
+int care_host_buffer_free(void* p) {
+    if (!p) return CARE_OK;
+    const hipError_t e = hipHostFree(p);
+    if (e != hipSuccess) return fail(CARE_ERR_DEVICE, "hipHostFree: %s", hipGetErrorString(e));
     return CARE_OK;
 }
 

@@ -407,6 +407,23 @@ HostOptions parseOptions(int argc, char** argv) {
     return o;
 }
 
+// Page-locked host array from the engine (care_host_buffer_alloc).
+template <class T>
+struct PinnedArray {
+    T* p = nullptr;
+    explicit PinnedArray(size_t n) {
+        void* v = nullptr;
+        if (care_host_buffer_alloc(n * sizeof(T), &v) != CARE_OK)
+            throw std::runtime_error(std::string("engine: ") + care_last_error());
+        p = static_cast<T*>(v);
+    }
+    ~PinnedArray() { care_host_buffer_free(p); }
+    PinnedArray(const PinnedArray&) = delete;
+    PinnedArray& operator=(const PinnedArray&) = delete;
+    T* data() { return p; }
+    T& operator[](size_t i) { return p[i]; }
+};
+
 // ---------------------------------------------------------------------------
 // Correction results (include/correctedsequence.hpp:22-146 semantics)
 // ---------------------------------------------------------------------------
@@ -603,11 +620,12 @@ int runMain(int argc, char** argv) {
     }
     const int pitch = (rs->maxLen + 15) / 16 * 16;
     const int chunk = 1 << 18;
-    std::vector<u8> status(chunk);
-    std::vector<char> seqs(size_t(chunk) * pitch);
+    // result buffers in page-locked memory: the engine copies into them by DMA
     const int candCap = o.eng.correct_candidates ? chunk * 64 : 0;
-    std::vector<u32> candIds(std::max(1, candCap));
-    std::vector<char> candSeqs(size_t(std::max(1, candCap)) * pitch);
+    PinnedArray<u8> status{static_cast<size_t>(chunk)};
+    PinnedArray<char> seqs{static_cast<size_t>(chunk) * pitch};
+    PinnedArray<u32> candIds{static_cast<size_t>(std::max(1, candCap))};
+    PinnedArray<char> candSeqs{static_cast<size_t>(std::max(1, candCap)) * pitch};
     ResultStore anchors, cands;
     std::vector<u8> hqAnchor(rs->n, 0);
     std::string original;

@@ -228,6 +228,14 @@ typedef struct care_batch_stats {
 } care_batch_stats;
 int care_last_batch_stats(care_context* ctx, care_batch_stats* stats);
 
+/* Page-locked host memory for result buffers (care_batch_result arrays):
+ * results are copied into it by DMA without staging through a bounce
+ * buffer.  Any host memory works as a result buffer; this only makes the
+ * copies faster.  The reference's counterpart is its pinned staging of
+ * results, helpers::SimpleAllocationPinnedHost (include/gpu/gpucorrector.cuh:115-126). */
+int care_host_buffer_alloc(size_t bytes, void** out);
+int care_host_buffer_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif
