@@ -65,6 +65,22 @@ WORKLOADS = {
 }
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-kernel calibrated), kept
+# with the commands that produced them in profiles/
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_correct_k_query_repeat-pe.txt")
+
+
+def measured_traffic(kernel, workload):
+    """Bytes per launch of `kernel` on `workload` from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                part = line.split()
+                if len(part) == 4 and part[0] == "traffic" and part[1] == kernel and part[2] == workload:
+                    return int(part[3])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -258,9 +274,10 @@ def main():
         alg = n * (pitch_bytes + w["max_length"] + 1) + r["candidates"] * (pitch_bytes + 4 + 16 + qbytes)
         launch_ms = r["correct_ms"] / r["launches"]
         achieved = (alg / r["launches"]) / (launch_ms / 1e3) / 1e9
+        kernel = "k_correct<%s>" % ("true" if r is gated else "false")
         return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_correct<%s>" % ("true" if r is gated else "false"), "launches_per_step": r["launches"],
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kernel, args.workload),
+                "kernel": kernel, "launches_per_step": r["launches"],
                 "bytes_per_launch": int(alg / r["launches"]), "avg_launch_ms": round(launch_ms, 4)}
 
     def roofline_query(r):
@@ -270,7 +287,8 @@ def main():
         launch_ms = r["query_ms"] / r["launches"]
         achieved = (alg / r["launches"]) / (launch_ms / 1e3) / 1e9
         return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "kernel": "k_query", "raw_hits_per_step": r["raw_hits"],
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("k_query", args.workload),
+                "kernel": "k_query", "raw_hits_per_step": r["raw_hits"],
                 "bytes_per_launch": int(alg / r["launches"]), "avg_launch_ms": round(launch_ms, 4)}
 
     cpu = None
