@@ -139,7 +139,10 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=Tru
     if batch:
         eng.set_launch_config(batch, 0)
     eng.upload_readset(rs)
+    tb = time.perf_counter()
     eng.build()
+    build_ms = (time.perf_counter() - tb) * 1e3
+    log("bench: tables built in %.1f ms (%d maps)" % (build_ms, opt.num_hash_maps))
     if gate is not None:
         eng.mlp_load(gate)
     n = rs.n
@@ -193,7 +196,7 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=Tru
     eng.close()
     return {"ms": sum(step_ms) / len(step_ms), "correct_ms": sum(corr_ms) / len(corr_ms),
             "query_ms": sum(query_ms) / len(query_ms), "launches": max(1, st.launches), "candidates": int(st.candidates),
-            "raw_hits": int(st.raw_hits), "h": int(opt.num_hash_maps), "tuning": tuning,
+            "raw_hits": int(st.raw_hits), "h": int(opt.num_hash_maps), "tuning": tuning, "build_ms": build_ms,
             "corrected": int(st.corrected)}
 
 
@@ -299,7 +302,8 @@ def main():
     def summary(r):
         return {"reads_per_s": round(n / (r["ms"] / 1e3), 1), "ms_per_step": round(r["ms"], 3),
                 "query_kernel_ms": round(r["query_ms"], 3), "correct_kernel_ms": round(r["correct_ms"], 3),
-                "candidates_per_step": r["candidates"], "corrected_anchors": r["corrected"], "launch_shapes": r["tuning"]}
+                "candidates_per_step": r["candidates"], "corrected_anchors": r["corrected"], "launch_shapes": r["tuning"],
+                "table_build_ms": round(r["build_ms"], 1)}
 
     config = {"workload": w["label"], "reads": n, "plain": summary(plain)}
     if gated is not None:

@@ -20,7 +20,10 @@ void launch_fill_u64(u64* p, u64 n, u64 v, hipStream_t s);
 void launch_fill_u32(u32* p, u64 n, u32 v, hipStream_t s);
 void launch_count_keys(const u64* sigs, u64 n, int h, int map, u64* tkeys, u32* tcount, u64 tmask, int* overflow,
                        hipStream_t s);
-void launch_assign_offsets(const u64* tkeys, const u32* tcount, u64 tsize, int cap, u32* toff,
+// value offsets of the keys with 2..cap values in slot order; tileSums holds
+// assign_offsets_scratch(tsize) words
+u64 assign_offsets_scratch(u64 tsize);
+void launch_assign_offsets(const u64* tkeys, const u32* tcount, u64 tsize, int cap, u32* toff, u64* tileSums,
                            unsigned long long* counters, hipStream_t s);
 void launch_insert_final(const u64* tkeys, const u32* tcount, const u32* toff, u64 tsize, u64* fkeys, uint2* foffcnt,
                          u64 fmask, int* overflow, hipStream_t s);
@@ -498,6 +501,8 @@ int care_minhash_build(care_context* ctx) {
         toff.alloc(tsize);
         DevBuf<unsigned long long> counters;
         counters.alloc(2);
+        DevBuf<u64> tileSums;
+        tileSums.alloc(assign_offsets_scratch(tsize));
         DevBuf<int> overflow;
         overflow.alloc(1);
         ctx->maps.clear();
@@ -509,7 +514,7 @@ int care_minhash_build(care_context* ctx) {
             launch_fill_u64(reinterpret_cast<u64*>(counters.p), 2, 0, s);
             launch_fill_u32(reinterpret_cast<u32*>(overflow.p), 1, 0, s);
             launch_count_keys(sigs.p, n, h, m, tkeys.p, tcount.p, tsize - 1, overflow.p, s);
-            launch_assign_offsets(tkeys.p, tcount.p, tsize, ctx->cap, toff.p, counters.p, s);
+            launch_assign_offsets(tkeys.p, tcount.p, tsize, ctx->cap, toff.p, tileSums.p, counters.p, s);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(ctx->hU64.p, counters.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));

@@ -147,33 +147,94 @@ __global__ void k_count_keys(const u64* sigs, u64 n, int h, int map, u64* tkeys,
     }
 }
 
-// assign value offsets to keys with 2..cap values; counters[0] = values,
-// counters[1] = valid keys
-__global__ void k_assign_offsets(const u64* tkeys, const u32* tcount, u64 tsize, int cap, u32* toff,
-                                 unsigned long long* counters) {
+// Value offsets of the keys with 2..cap values, in slot order: every block
+// sums a tile of kTile slots (values, keys), one block scans the tile sums,
+// then every block assigns the offsets of its tile.  Deterministic, and no
+// same-address atomics.  counters[0] = values, counters[1] = valid keys.
+constexpr int kTile = 4096;
+
+__device__ __forceinline__ u32 slot_values(const u64* tkeys, const u32* tcount, u64 s, u64 tsize, int cap) {
+    if (s >= tsize || tkeys[s] == kEmptyKey) return 0;
+    const u32 c = tcount[s];
+    return (c >= 2 && c <= u32(cap)) ? c : 0;
+}
+
+// exclusive prefix sum of v over the block in thread order; *total gets the
+// block's sum.  scratch: blockDim / 64 + 1 u64.  All threads must call.
+__device__ __forceinline__ u64 block_exclusive_sum(u64 v, u64* scratch, u64* total) {
     const int lane = lane_id();
-    for (u64 base = blockIdx.x * u64(blockDim.x); base < tsize; base += u64(gridDim.x) * blockDim.x) {
-        const u64 s = base + threadIdx.x;
-        u32 cnt = 0;
-        if (s < tsize && tkeys[s] != kEmptyKey) {
-            const u32 c = tcount[s];
-            if (c >= 2 && c <= u32(cap)) cnt = c;
-        }
-        // wave-aggregated offset allocation
-        u32 incl = cnt;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const u32 v = __shfl_up(incl, o, kWave);
-            if (lane >= o) incl += v;
-        }
-        const u32 waveTotal = __shfl(incl, kWave - 1, kWave);
-        const unsigned long long valid = __ballot(cnt > 0);
-        unsigned long long waveBase = 0;
-        if (lane == 0 && waveTotal > 0) {
-            waveBase = atomicAdd(&counters[0], (unsigned long long)waveTotal);
-            atomicAdd(&counters[1], (unsigned long long)__popcll(valid));
-        }
-        waveBase = __shfl(waveBase, 0, kWave);
-        if (s < tsize) toff[s] = cnt > 0 ? u32(waveBase + incl - cnt) : 0xFFFFFFFFu;
+    const int wave = threadIdx.x / kWave;
+    const int nwaves = (blockDim.x + kWave - 1) / kWave;
+    u64 incl = v;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const u64 t = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += t;
+    }
+    __syncthreads();
+    if (lane == kWave - 1) scratch[wave] = incl;
+    __syncthreads();
+    u64 before = 0, all = 0;
+    for (int w = 0; w < nwaves; w++) {
+        const u64 c = scratch[w];
+        if (w < wave) before += c;
+        all += c;
+    }
+    *total = all;
+    return before + incl - v;
+}
+
+__global__ void __launch_bounds__(256) k_tile_sums(const u64* tkeys, const u32* tcount, u64 tsize, int cap,
+                                                   u64* tileSums) {
+    __shared__ u64 scr[8];
+    const u64 t0 = u64(blockIdx.x) * kTile;
+    u64 vals = 0, keys = 0;
+    for (int i = threadIdx.x; i < kTile; i += blockDim.x) {
+        const u32 c = slot_values(tkeys, tcount, t0 + u64(i), tsize, cap);
+        vals += c;
+        keys += c > 0;
+    }
+    u64 tv, tk;
+    block_exclusive_sum(vals, scr, &tv);
+    block_exclusive_sum(keys, scr, &tk);
+    if (threadIdx.x == 0) {
+        tileSums[2 * u64(blockIdx.x)] = tv;
+        tileSums[2 * u64(blockIdx.x) + 1] = tk;
+    }
+}
+
+// one block: tileSums[2t] becomes the first value offset of tile t
+__global__ void __launch_bounds__(1024) k_scan_tiles(u64* tileSums, u64 nt, unsigned long long* counters) {
+    __shared__ u64 scr[17];
+    u64 carryV = 0, carryK = 0;
+    for (u64 b = 0; b < nt; b += blockDim.x) {
+        const u64 t = b + threadIdx.x;
+        const u64 v = t < nt ? tileSums[2 * t] : 0;
+        const u64 k = t < nt ? tileSums[2 * t + 1] : 0;
+        u64 sv, sk;
+        const u64 ev = block_exclusive_sum(v, scr, &sv);
+        block_exclusive_sum(k, scr, &sk);
+        if (t < nt) tileSums[2 * t] = carryV + ev;
+        carryV += sv;
+        carryK += sk;
+    }
+    if (threadIdx.x == 0) {
+        counters[0] = carryV;
+        counters[1] = carryK;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_tile_offsets(const u64* tkeys, const u32* tcount, u64 tsize, int cap,
+                                                      const u64* tileSums, u32* toff) {
+    __shared__ u64 scr[8];
+    const u64 t0 = u64(blockIdx.x) * kTile;
+    u64 run = tileSums[2 * u64(blockIdx.x)];
+    for (int r = 0; r < kTile; r += blockDim.x) {
+        const u64 s = t0 + u64(r) + threadIdx.x;
+        const u32 c = slot_values(tkeys, tcount, s, tsize, cap);
+        u64 sum;
+        const u64 ex = block_exclusive_sum(c, scr, &sum);
+        if (s < tsize) toff[s] = c > 0 ? u32(run + ex) : 0xFFFFFFFFu;
+        run += sum;
     }
 }
 
@@ -241,10 +302,14 @@ void launch_count_keys(const u64* sigs, u64 n, int h, int map, u64* tkeys, u32* 
     hipLaunchKernelGGL(k_count_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, sigs, n, h, map, tkeys, tcount, tmask,
                        overflow);
 }
-void launch_assign_offsets(const u64* tkeys, const u32* tcount, u64 tsize, int cap, u32* toff,
+u64 assign_offsets_scratch(u64 tsize) { return 2 * ((tsize + kTile - 1) / kTile); }
+
+void launch_assign_offsets(const u64* tkeys, const u32* tcount, u64 tsize, int cap, u32* toff, u64* tileSums,
                            unsigned long long* counters, hipStream_t s) {
-    hipLaunchKernelGGL(k_assign_offsets, dim3(grid_for(tsize, 256)), dim3(256), 0, s, tkeys, tcount, tsize, cap, toff,
-                       counters);
+    const u64 nt = (tsize + kTile - 1) / kTile;
+    hipLaunchKernelGGL(k_tile_sums, dim3(unsigned(nt)), dim3(256), 0, s, tkeys, tcount, tsize, cap, tileSums);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tileSums, nt, counters);
+    hipLaunchKernelGGL(k_tile_offsets, dim3(unsigned(nt)), dim3(256), 0, s, tkeys, tcount, tsize, cap, tileSums, toff);
 }
 void launch_insert_final(const u64* tkeys, const u32* tcount, const u32* toff, u64 tsize, u64* fkeys, uint2* foffcnt,
                          u64 fmask, int* overflow, hipStream_t s) {
