@@ -349,8 +349,8 @@ int care_set_tuning(care_context* ctx, const care_tuning* t) {
     if (4 * query_wave_lds_bytes(q, ctx->opt.candidate_limit) > size_t(kMaxLdsBytes))
         return fail(CARE_ERR_CAPACITY, "query_lds_hits %d exceeds the query workgroup's LDS", q);
     const int rm = t->query_register_hits;
-    if (rm != 0 && (rm < 64 || rm > 1024 || (rm & (rm - 1))))
-        return fail(CARE_ERR_ARGUMENT, "query_register_hits must be 0 or a power of two in 64..1024");
+    if (rm != 0 && (rm < 64 || rm > 2048 || (rm & (rm - 1))))
+        return fail(CARE_ERR_ARGUMENT, "query_register_hits must be 0 or a power of two in 64..2048");
     ctx->requestedBatch = t->batch_anchors;
     ctx->blockThreads = t->correct_threads ? t->correct_threads : 256;
     ctx->blocksPerCU = t->correct_blocks_per_cu;
@@ -370,7 +370,9 @@ int care_get_tuning(care_context* ctx, care_tuning* out) {
     out->msa_chunk = correct_msa_chunk(ctx->rs.maxLen ? ctx->rs.maxLen : 1, o.use_quality, ctx->rs.qpitch, T, gatedK,
                                        ctx->msaChunk);
     out->query_lds_hits = ctx->queryCap;
-    out->query_register_hits = ctx->queryRegMax ? ctx->queryRegMax : 1024;
+    // the query kernel sorts up to 1024 hits in registers, 2048 when lists can be longer
+    const int regLimit = ctx->maxCand > 1024 ? 2048 : 1024;
+    out->query_register_hits = ctx->queryRegMax ? std::min(ctx->queryRegMax, regLimit) : regLimit;
     out->correct_blocks_per_cu = ctx->blocksPerCU;
     if (ctx->blocksPerCU == 0 && ctx->rs.maxLen) {
         try {

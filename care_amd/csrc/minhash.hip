@@ -503,17 +503,19 @@ __device__ __forceinline__ void gather_sort(Buf out, const int* counts, const u3
     wave_sync_mem();
 }
 
-// longest hit list sorted in registers (16 per lane: longer networks cost
-// more registers than they save); longer lists are merged
-constexpr int kRegisterSortMax = 16 * kWave;
+// longest hit list sorted in registers: 16 per lane in the kernel for tables
+// whose lists stay short (h * cap <= 1024), 32 per lane otherwise (more
+// registers, fewer lists left to the merge); longer lists are merged
+constexpr int kRegisterSortMax = 32 * kWave;
 
-template <class Buf>
+template <int kMaxK, class Buf>
 __device__ __forceinline__ void gather_sort_any(Buf out, const int* counts, const u32* const* lists, int h, int total,
                                                 int lane) {
     if (total <= 2 * kWave) gather_sort<2>(out, counts, lists, h, total, lane);
     else if (total <= 4 * kWave) gather_sort<4>(out, counts, lists, h, total, lane);
     else if (total <= 8 * kWave) gather_sort<8>(out, counts, lists, h, total, lane);
-    else gather_sort<16>(out, counts, lists, h, total, lane);
+    else if (kMaxK == 16 || total <= 16 * kWave) gather_sort<16>(out, counts, lists, h, total, lane);
+    else gather_sort<(kMaxK > 16 ? kMaxK : 16)>(out, counts, lists, h, total, lane);
 }
 
 // unique, self/ambiguous removal and compaction into out; hit counts
@@ -603,6 +605,7 @@ __device__ __forceinline__ int keep_top(u32* out, HitBuf hits, int written, int 
 
 }  // namespace
 
+template <int kMaxK>
 __global__ void __launch_bounds__(256) k_query(DevReadStore rs, DevTables t, u64 first, int n, int maxCand, int capL,
                                                int regMax, int excludeAmbiguous, int candLimit, u32* candIds,
                                                int* candCount, u32* spill, int* rawCount) {
@@ -660,12 +663,12 @@ __global__ void __launch_bounds__(256) k_query(DevReadStore rs, DevTables t, u64
         int written;
         if (total <= capL) {
             const u32* sorted = la;
-            if (total <= regMax) gather_sort_any(la, counts, lists, h, total, lane);
+            if (total <= regMax) gather_sort_any<kMaxK>(la, counts, lists, h, total, lane);
             else sorted = gather_merge(la, lb, counts, lists, h, total, lane);
             written = compact_unique(sorted, total, u32(id), rs, excludeAmbiguous, candLimit, out, lh, lane);
         } else {
             const u32* sorted = gs;
-            if (total <= regMax) gather_sort_any(gs, counts, lists, h, total, lane);
+            if (total <= regMax) gather_sort_any<kMaxK>(gs, counts, lists, h, total, lane);
             else sorted = gather_merge(gs, gs + maxCand, counts, lists, h, total, lane);
             written = compact_unique(sorted, total, u32(id), rs, excludeAmbiguous, candLimit, out, gh, lane);
         }
@@ -727,9 +730,15 @@ void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, 
     const int wpb = threads / kWave;
     const int lds = query_lds_bytes(t.h, maxCand / t.h, rs.maxLen, threads, candLimit, capL);
     const int blocks = std::min((n + wpb - 1) / wpb, 1 << 16);
-    regMax = regMax > 0 ? std::min(regMax, kRegisterSortMax) : kRegisterSortMax;
-    hipLaunchKernelGGL(k_query, dim3(blocks), dim3(threads), lds, stream, rs, t, first, n, maxCand, capL, regMax,
-                       excludeAmbiguous, candLimit, candIds, candCount, spill, rawCount);
+    const bool wide = maxCand > 16 * kWave;  // lists may exceed the 16-per-lane network
+    const int kernelMax = wide ? 32 * kWave : 16 * kWave;
+    regMax = regMax > 0 ? std::min(regMax, kernelMax) : kernelMax;
+    if (wide)
+        hipLaunchKernelGGL(k_query<32>, dim3(blocks), dim3(threads), lds, stream, rs, t, first, n, maxCand, capL, regMax,
+                           excludeAmbiguous, candLimit, candIds, candCount, spill, rawCount);
+    else
+        hipLaunchKernelGGL(k_query<16>, dim3(blocks), dim3(threads), lds, stream, rs, t, first, n, maxCand, capL, regMax,
+                           excludeAmbiguous, candLimit, candIds, candCount, spill, rawCount);
 }
 
 void launch_sum_counts(const int* rawCount, const int* candCount, int n, unsigned long long* totals, hipStream_t s) {

@@ -148,8 +148,9 @@ typedef struct care_tuning {
                                    multiple of 16 in 16..4096; longer lists use global
                                    scratch (0 = 512) */
     int query_register_hits;    /* longest hit list sorted in registers (bitonic network
-                                   across the wavefront), power of two in 64..1024;
-                                   longer lists are merged list by list (0 = 1024) */
+                                   across the wavefront), power of two in 64..2048;
+                                   longer lists are merged list by list (0 = 1024, or
+                                   2048 when num_hash_maps * bucket cap exceeds 1024) */
 } care_tuning;
 
 int care_set_tuning(care_context* ctx, const care_tuning* tuning);

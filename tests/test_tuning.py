@@ -128,7 +128,7 @@ def test_tuning_rejects_invalid_shapes(emu, tmp_path):
     eng.upload_readset(rs)
     for bad in (dict(correct_threads=64), dict(correct_threads=512), dict(msa_chunk=8), dict(query_lds_hits=24),
                 dict(query_lds_hits=8192), dict(batch_anchors=3), dict(query_register_hits=100),
-                dict(query_register_hits=2048)):
+                dict(query_register_hits=4096)):
         with pytest.raises(care_amd.CareError):
             eng.set_tuning(**bad)
     with pytest.raises(care_amd.CareError):
