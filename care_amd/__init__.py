@@ -141,6 +141,11 @@ ENGINE_SYMBOLS = {
     "care_gate_columns": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "care_gate_columns_candidates": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_uint64,
+                                                    ctypes.POINTER(ctypes.c_uint64)]),
+    "care_mlp_load_candidates": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 HOST_SYMBOLS = {
@@ -433,9 +438,16 @@ class Engine:
         self._ck(self.lib.care_mlp_infer(self.ctx, x.ctypes.data, len(x), p.ctypes.data, z.ctypes.data))
         return z
 
-    def gate_columns(self, first, n, capacity=None):
+    def mlp_load_candidates(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.shape == (MLP_PARAMS,)
+        self._ck(self.lib.care_mlp_load_candidates(self.ctx, p.ctypes.data))
+
+    def gate_columns(self, first, n, capacity=None, candidates=False):
         """Decision columns of the gated mode: features [c][32], read ids,
-        positions, original and consensus bases."""
+        positions, original and consensus bases (candidates: those of the
+        candidate corrections, forward-strand positions and bases)."""
+        fn = self.lib.care_gate_columns_candidates if candidates else self.lib.care_gate_columns
         cap = capacity if capacity is not None else max(1, n) * 8
         while True:
             f = np.zeros((cap, MLP_FEATURES), dtype=np.float32)
@@ -444,8 +456,8 @@ class Engine:
             orig = np.zeros(cap, dtype=np.uint8)
             cons = np.zeros(cap, dtype=np.uint8)
             cnt = ctypes.c_uint64()
-            rc = self.lib.care_gate_columns(self.ctx, first, n, f.ctypes.data, ids.ctypes.data, pos.ctypes.data,
-                                            orig.ctypes.data, cons.ctypes.data, cap, ctypes.byref(cnt))
+            rc = fn(self.ctx, first, n, f.ctypes.data, ids.ctypes.data, pos.ctypes.data, orig.ctypes.data,
+                    cons.ctypes.data, cap, ctypes.byref(cnt))
             if rc == 4 and capacity is None:
                 cap *= 4
                 continue

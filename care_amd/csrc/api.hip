@@ -177,7 +177,7 @@ struct care_context {
         DevBuf<float> params;
         DevBuf<unsigned short> w1, w2, w3;
         bool loaded = false;
-    } gate;
+    } gate, gateCands;  // anchor columns; candidate columns (defaults to the anchor gate)
     struct GateRecord {
         DevBuf<float> feat;
         DevBuf<u32> ids;
@@ -185,6 +185,7 @@ struct care_context {
         DevBuf<u8> orig, cons;
         int cap = 0;
         bool on = false;
+        bool cands = false;  // record candidate decision columns instead of the anchors'
     } record;
 };
 
@@ -271,8 +272,10 @@ int care_context_create(int device, const care_options* opt, care_context** out)
     if (opt->num_hash_maps < 1 || opt->num_hash_maps > 64)
         return fail(CARE_ERR_ARGUMENT, "num_hash_maps %d outside 1..64", opt->num_hash_maps);
     if ((opt->correction_type != 0 && opt->correction_type != 1 && opt->correction_type != 3) ||
-        (opt->correction_type_cands != 0 && opt->correction_type_cands != 1))
-        return fail(CARE_ERR_ARGUMENT, "correction types: 0 classic, 1 forest, 3 learned gate (anchors only)");
+        (opt->correction_type_cands != 0 && opt->correction_type_cands != 1 && opt->correction_type_cands != 3))
+        return fail(CARE_ERR_ARGUMENT, "correction types: 0 classic, 1 forest, 3 learned gate");
+    if (opt->correction_type_cands == 3 && opt->correction_type != 3)
+        return fail(CARE_ERR_ARGUMENT, "correction_type_cands 3 (learned gate) needs correction_type 3");
     if (opt->candidate_limit < 0) return fail(CARE_ERR_ARGUMENT, "candidate_limit must be >= 0");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CARE_ERR_DEVICE, "no HIP device available");
@@ -364,7 +367,7 @@ int care_get_tuning(care_context* ctx, care_tuning* out) {
     if (!ctx || !out) return fail(CARE_ERR_ARGUMENT, "null argument");
     const auto& o = ctx->opt;
     const int T = ctx->blockThreads;
-    const int gatedK = o.correction_type == 3;
+    const int gatedK = correct_gate_mode(o.correction_type, o.correct_candidates, o.correction_type_cands);
     out->batch_anchors = ctx->rs.n ? default_batch(ctx) : ctx->requestedBatch;
     out->correct_threads = T;
     out->msa_chunk = correct_msa_chunk(ctx->rs.maxLen ? ctx->rs.maxLen : 1, o.use_quality, ctx->rs.qpitch, T, gatedK,
@@ -670,6 +673,19 @@ int prepare_params(care_context* ctx, CorrectParams& P) {
             P.gateCons = ctx->record.cons.p;
             P.gateCount = ctx->counters.p + 2;
             P.gateCap = ctx->record.cap;
+            P.gateRecordCands = ctx->record.cands ? 1 : 0;
+        }
+        if (o.correct_candidates && o.correction_type_cands == 3) {
+            // the candidate gate's own weights if loaded, else the anchor gate's
+            const auto& gc = ctx->gateCands.loaded ? ctx->gateCands : ctx->gate;
+            if (gc.loaded) {
+                P.gateCW1 = gc.w1.p;
+                P.gateCW2 = gc.w2.p;
+                P.gateCW3 = gc.w3.p;
+                P.gateCParams = gc.params.p;
+            }
+            const float tc = std::min(std::max(o.threshold_cands, 1e-6f), 1.0f - 1e-6f);
+            P.gateCLogit = std::log(tc / (1.0f - tc));
         }
     }
     if (o.correct_candidates && o.correction_type_cands == 1) {
@@ -698,7 +714,7 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
         const int B = default_batch(ctx);
         ensure_batch(ctx, B);
         const int T = ctx->blockThreads;
-        const int gatedK = o.correction_type == 3;
+        const int gatedK = correct_gate_mode(o.correction_type, o.correct_candidates, o.correction_type_cands);
         const size_t lds = correct_lds_bytes(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T, gatedK, ctx->msaChunk);
         if (lds > size_t(kMaxLdsBytes))
             return fail(CARE_ERR_CAPACITY, "reads of length %d need %zu bytes of LDS per correction workgroup (limit %d)",
@@ -950,11 +966,22 @@ int care_forest_load(care_context* ctx, int which, const char* path, uint32_t ma
 // Learned gate: parameters, training, inference, recorded decision columns
 // ---------------------------------------------------------------------------
 
-int care_mlp_load(care_context* ctx, const float* params) {
+namespace {
+int load_gate(care_context* ctx, const float* params, bool cands);
+}
+
+int care_mlp_load(care_context* ctx, const float* params) { return load_gate(ctx, params, false); }
+
+int care_mlp_load_candidates(care_context* ctx, const float* params) { return load_gate(ctx, params, true); }
+
+}  // extern "C"
+
+namespace {
+int load_gate(care_context* ctx, const float* params, bool cands) {
     if (!ctx || !params) return fail(CARE_ERR_ARGUMENT, "null argument");
     for (int i = 0; i < CARE_MLP_PARAMS; i++)
         if (!std::isfinite(params[i])) return fail(CARE_ERR_ARGUMENT, "parameter %d is not finite", i);
-    auto& g = ctx->gate;
+    auto& g = cands ? ctx->gateCands : ctx->gate;
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         g.params.alloc(CARE_MLP_PARAMS);
@@ -971,6 +998,9 @@ int care_mlp_load(care_context* ctx, const float* params) {
     g.loaded = true;
     return CARE_OK;
 }
+}  // namespace
+
+extern "C" {
 
 extern "C++" {
 namespace {
@@ -1152,11 +1182,32 @@ int care_mlp_infer(care_context* ctx, const float* features, uint64_t n, const f
     return CARE_OK;
 }
 
+namespace {
+int gate_columns_impl(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
+                      uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count,
+                      bool cands);
+}
+
 int care_gate_columns(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
                       uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count) {
+    return gate_columns_impl(ctx, first, n, features, read_ids, positions, orig, cons, capacity, count, false);
+}
+
+int care_gate_columns_candidates(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
+                                 uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity,
+                                 uint64_t* count) {
+    return gate_columns_impl(ctx, first, n, features, read_ids, positions, orig, cons, capacity, count, true);
+}
+
+}  // extern "C"
+
+namespace {
+int gate_columns_impl(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
+                      uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count,
+                      bool cands) {
     if (!ctx || !features || !read_ids || !positions || !orig || !cons || !count)
         return fail(CARE_ERR_ARGUMENT, "null argument");
-    if (ctx->opt.correction_type != 3) return fail(CARE_ERR_STATE, "care_gate_columns needs correction_type 3");
+    if (ctx->opt.correction_type != 3) return fail(CARE_ERR_STATE, "gate columns need correction_type 3");
     if (capacity < 1 || capacity > (uint64_t(1) << 31)) return fail(CARE_ERR_ARGUMENT, "capacity out of range");
     *count = 0;
     auto& r = ctx->record;
@@ -1186,10 +1237,25 @@ int care_gate_columns(care_context* ctx, uint64_t first, int n, float* features,
     res.anchor_status = status.data();
     res.anchor_sequences = seqs.data();
     res.sequence_pitch = pitch;
-    const bool cc = ctx->opt.correct_candidates != 0;
-    ctx->opt.correct_candidates = 0;
+    // anchor columns: no candidate correction; candidate columns: candidate
+    // correction through the gate, results discarded
+    const int cc = ctx->opt.correct_candidates, ctc = ctx->opt.correction_type_cands;
+    ctx->opt.correct_candidates = cands ? 1 : 0;
+    ctx->opt.correction_type_cands = cands ? 3 : ctc;
+    std::vector<u32> candIds;
+    std::vector<char> candSeqs;
+    if (cands) {
+        res.candidate_capacity = std::max(n, 1) * 64;
+        candIds.resize(size_t(res.candidate_capacity));
+        candSeqs.resize(size_t(res.candidate_capacity) * size_t(pitch));
+        res.candidate_read_ids = candIds.data();
+        res.candidate_sequences = candSeqs.data();
+    }
+    r.cands = cands;
     const int rc = correct_batch_impl(ctx, first, n, &res);
     ctx->opt.correct_candidates = cc;
+    ctx->opt.correction_type_cands = ctc;
+    r.cands = false;
     r.on = false;
     r.cap = savedCap;
     if (rc != CARE_OK) return rc;
@@ -1210,6 +1276,9 @@ int care_gate_columns(care_context* ctx, uint64_t first, int n, float* features,
     }
     return CARE_OK;
 }
+}  // namespace
+
+extern "C" {
 
 // Shape search: the query kernel over the LDS hit capacities that fit,
 // then the correction kernel over workgroup sizes and MSA staging chunks,
@@ -1288,7 +1357,7 @@ int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen)
         P.queueHead = ctx->counters.p;
         P.firstAnchor = first;
         P.numAnchors = n;
-        const int gatedK = o.correction_type == 3;
+        const int gatedK = correct_gate_mode(o.correction_type, o.correct_candidates, o.correction_type_cands);
         int bestT = ctx->blockThreads, bestChunk = ctx->msaChunk;
         float bestC = 1e30f;
         for (int T : {256, 128}) {

@@ -94,7 +94,7 @@ struct LdsDims {
         b += sizeof(float) * cols + 2 * sizeof(int) * cols;
         b += 4 * sizeof(u16) * cols + 4 * sizeof(float) * cols;
         b += 32 * sizeof(int);
-        if (gated) b += (sizeof(u16) + 1) * ((size_t(maxLen) + 15) / 16 * 16);
+        if (gated & 1) b += (sizeof(u16) + 1) * ((size_t(maxLen) + 15) / 16 * 16);
         return b;
     }
 };
@@ -119,7 +119,7 @@ __device__ Lds carve(char* base, const LdsDims& d) {
     l.cons = reinterpret_cast<u8*>(p); p += (size_t(d.cols) + 3) / 4 * 4;
     l.anchorQ = reinterpret_cast<char*>(p); p += (size_t(d.maxLen) + 3) / 4 * 4;
     l.gscr = reinterpret_cast<char*>(l.mst);
-    if (d.gated) {
+    if (d.gated & 1) {
         const size_t ml = (size_t(d.maxLen) + 15) / 16 * 16;
         l.dec = reinterpret_cast<u16*>(p); p += sizeof(u16) * ml;
         l.gacc = reinterpret_cast<u8*>(p);
@@ -142,8 +142,9 @@ struct Msa {
 // chunks of dims.chunk candidates.  Only the columns within `margin` of the
 // anchor are computed: every reader of the MSA (refinement, properties,
 // anchor correction and its features, candidate correction within
-// new_columns of the anchor) stays inside them, and columns are independent
-// of each other.  All threads must call.
+// new_columns of the anchor and the gate features two columns around a
+// corrected read) stays inside them, and columns are independent of each
+// other.  All threads must call.
 __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& dims, const CandMeta* meta, int m,
                          int la, int margin, int* sMin, int* sMax) {
     if (threadIdx.x == 0) {
@@ -457,14 +458,18 @@ __device__ void column_features(const Lds& L, int cols, int pos, u32 origBase, c
     f[12] = float(cs.minCov) / cov;
 }
 
-// The 32 gate features of anchor position i (column ab + i) whose
-// consensus differs from the anchor base.  `acs` are the anchor-column MSA
-// properties, nd the number of decision columns of the anchor, m the
-// candidates in the MSA.  Shared with the oracle's restatement.
+// The 32 gate features of position i of a read in the MSA (column ab + i)
+// whose consensus differs from the read's base: the anchor, or a candidate
+// in its MSA orientation.  base(ii) is the read's base at position ii, qwi
+// the quality weight of its base i (1 without qualities), `acs` the MSA
+// properties over the read's columns, nd the read's number of decision
+// columns, m the candidates in the MSA.  Shared with the oracle's
+// restatement.
+template <class BaseAt>
 __device__ void gate_features(const Lds& L, int cols, int ab, int la, int i, int nd, int m, const ColumnStats& acs,
-                              float estCov, bool useQ, float* f) {
+                              float estCov, float qwi, const BaseAt& base, float* f) {
     const int col = ab + i;
-    const u32 o = base_at(L.aw, i);
+    const u32 o = base(i);
     const u32 c = L.cons[col];
     const float cov = float(L.coverage[col]);
     const float w0 = L.wgt[0 * cols + col], w1 = L.wgt[1 * cols + col], w2 = L.wgt[2 * cols + col],
@@ -485,7 +490,7 @@ __device__ void gate_features(const Lds& L, int cols, int ab, int la, int i, int
     f[6] = cov / estCov;
     f[7] = wsum / estCov;
     f[8] = (cov - no - nc) / cov;
-    f[9] = useQ ? L.qw[(unsigned char)L.anchorQ[i]] : 1.0f;
+    f[9] = qwi;
     f[10] = float(i) / float(la);
     f[11] = float(min(i, la - 1 - i)) / float(la);
     f[12] = acs.avgSupport;
@@ -498,7 +503,7 @@ __device__ void gate_features(const Lds& L, int cols, int ab, int la, int i, int
         const int cc = col + dcol[k];
         f[17 + k] = (cc >= 0 && cc < cols) ? L.support[cc] : 0.0f;
         const int ii = i + dcol[k];
-        f[21 + k] = (ii >= 0 && ii < la && u32(L.cons[ab + ii]) != base_at(L.aw, ii)) ? 1.0f : 0.0f;
+        f[21 + k] = (ii >= 0 && ii < la && u32(L.cons[ab + ii]) != base(ii)) ? 1.0f : 0.0f;
     }
     f[25] = float(nd) / float(la);
     f[26] = col - 1 >= 0 ? float(L.coverage[col - 1]) / estCov : 0.0f;
@@ -510,16 +515,98 @@ __device__ void gate_features(const Lds& L, int cols, int ab, int la, int i, int
     int near = 0;
     for (int d = -5; d <= 5; d++) {
         const int ii = i + d;
-        if (d != 0 && ii >= 0 && ii < la && u32(L.cons[ab + ii]) != base_at(L.aw, ii)) near++;
+        if (d != 0 && ii >= 0 && ii < la && u32(L.cons[ab + ii]) != base(ii)) near++;
     }
     f[31] = float(near) / 10.0f;
 }
 
+// Per-wave LDS of the candidate gate: feature rows f32 [16][33], input tile
+// bf16 [16][32], logits [16], MLP scratch, then one accept flag per position.
+constexpr int kCandGateTile = 16 * (kMlpIn + 1) * 4 + 16 * kMlpIn * 2 + 16 * 4 + kMlpTileScratch;
+__host__ __device__ inline int cand_gate_scratch(int maxLen) { return kCandGateTile + (maxLen + 15) / 16 * 16; }
+
+// Learned gate over the decision columns of one candidate (positions where
+// the consensus differs from its base), 16 at a time in position order, by
+// one wavefront; acc[i] = 1 if the gate accepts the consensus at i.  Decision
+// columns are recorded when the engine records candidate columns.
+__device__ void gate_candidate(const CorrectParams& P, const Lds& L, const DevReadStore& rs, int cols, int qb,
+                               const CandMeta& cm, int m, const ColumnStats& ccs, char* ws, u8* acc) {
+    const int lane = lane_id();
+    const int len = cm.len;
+    const bool rc = cm.flag == 2;
+    const u32* enc = rs.seqs + u64(cm.id) * rs.pitch;
+    const u32* qrow = P.useQuality ? rs.quals + u64(cm.id) * rs.qpitch : nullptr;
+    auto base = [&](int ii) { return oriented_base(enc, len, ii, rc); };
+    float* ft = reinterpret_cast<float*>(ws);                                      // [16][33]
+    bf16raw* xt = reinterpret_cast<bf16raw*>(ws + 16 * (kMlpIn + 1) * 4);          // [16][32]
+    float* zt = reinterpret_cast<float*>(ws + 16 * (kMlpIn + 1) * 4 + 16 * kMlpIn * 2);  // [16]
+    char* scr = ws + 16 * (kMlpIn + 1) * 4 + 16 * kMlpIn * 2 + 16 * 4;
+    const u64 below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int nd = 0;
+    for (int i0 = 0; i0 < len; i0 += kWave) {
+        const int i = i0 + lane;
+        const bool d = i < len && u32(L.cons[qb + i]) != base(i);
+        nd += __popcll(__ballot(d));
+        if (i < len) acc[i] = 0;
+    }
+    wave_sync();
+    MlpWeights W;
+    W.w1 = P.gateCW1;
+    W.w2 = P.gateCW2;
+    W.w3 = P.gateCW3;
+    if (P.gateCParams) {
+        W.b1 = P.gateCParams + kMlpB1;
+        W.b2 = P.gateCParams + kMlpB2;
+        W.b3 = P.gateCParams + kMlpB3;
+        W.w4 = P.gateCParams + kMlpW4;
+        W.b4 = P.gateCParams[kMlpB4];
+    }
+    for (int i0 = 0; i0 < len; i0 += kWave) {
+        const int i = i0 + lane;
+        const bool d = i < len && u32(L.cons[qb + i]) != base(i);
+        u64 bal = __ballot(d);
+        while (bal) {  // uniform: bal is the same in every lane
+            const int ng = min(16, __popcll(bal));
+            const int rank = __popcll(bal & below);
+            const bool mine = ((bal >> lane) & 1ull) && rank < 16;
+            if (mine) {
+                float* f = ft + rank * (kMlpIn + 1);
+                const float qwi =
+                    qrow ? L.qw[(unsigned char)quality_char(qrow, rc ? rs.maxLen - 1 - i : i, rs.qbits)] : 1.0f;
+                gate_features(L, cols, qb, len, i, nd, m, ccs, P.estimatedCoverage, qwi, base, f);
+                for (int k = 0; k < kMlpIn; k++) xt[rank * kMlpIn + k] = f2bf(f[k]);
+                if (P.gateFeat && P.gateRecordCands) {
+                    const int slot = atomicAdd(P.gateCount, 1);
+                    if (slot < P.gateCap) {
+                        // forward-strand position and bases of the candidate read
+                        const u32 o = base(i), cn = L.cons[qb + i];
+                        for (int k = 0; k < kMlpIn; k++) P.gateFeat[u64(slot) * kMlpIn + k] = f[k];
+                        P.gateIds[slot] = cm.id;
+                        P.gatePos[slot] = u16(rc ? len - 1 - i : i);
+                        P.gateOrig[slot] = u8(rc ? 3u - o : o);
+                        P.gateCons[slot] = u8(rc ? 3u - cn : cn);
+                    }
+                }
+            }
+            if (lane < 16 && lane >= ng)
+                for (int k = 0; k < kMlpIn; k++) xt[lane * kMlpIn + k] = 0;
+            wave_sync();
+            if (P.gateCParams) mlp_tile_forward(W, xt, scr, zt);
+            wave_sync();
+            if (mine) acc[i] = (P.gateCParams && zt[rank] >= P.gateCLogit) ? 1 : 0;
+            wave_sync();
+            for (int k = 0; k < ng; k++) bal &= bal - 1;
+        }
+    }
+    wave_sync();
+}
+
 }  // namespace
 
-// kGated: the learned-gate mode (correctionType 3), compiled separately so
-// that its register use does not lower the occupancy of the other modes.
-template <bool kGated>
+// kGated: the learned-gate mode (correctionType 3), kGateCands: with the
+// gate on candidate columns too (correctionTypeCands 3); compiled separately
+// so that their register use does not lower the occupancy of the other modes.
+template <bool kGated, bool kGateCands>
 __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) {
     HIP_DYNAMIC_SHARED(uint4, ldsRaw)  // uint4: 16-byte aligned dynamic LDS
     const Lds L = carve(reinterpret_cast<char*>(ldsRaw), dims);
@@ -770,9 +857,9 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         // ---- phase C: MSA + refinement (include/corrector.hpp:1307-1421) ----
         CandMeta* cur = metaB;
         CandMeta* other = metaA;
-        // MSA columns read beyond the anchor: neighbours of gate features (2),
-        // candidate correction (new_columns)
-        const int margin = P.correctCandidates ? max(2, P.newColumns) : 2;
+        // MSA columns read beyond the anchor: candidate correction reaches
+        // new_columns past it, gate features two columns past a corrected read
+        const int margin = P.correctCandidates ? P.newColumns + 2 : 2;
         Msa msa = build_msa(P, L, dims, cur, m, la, margin, &sMin, &sMax);
         const int maxIter = kGated ? 0 : 5;  // the gated mode skips refinement
         for (int iter = 0; iter < maxIter && m > 0; iter++) {
@@ -910,13 +997,15 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     if (tid < 32) {
                         float* f = ft + tid * (kMlpIn + 1);
                         if (tid < ng) {
-                            gate_features(L, msa.cols, msa.ab, la, int(L.dec[g0 + tid]), nd, m, acs,
-                                          P.estimatedCoverage, P.useQuality != 0, f);
+                            const int i = int(L.dec[g0 + tid]);
+                            const float qwi = P.useQuality ? L.qw[(unsigned char)L.anchorQ[i]] : 1.0f;
+                            gate_features(L, msa.cols, msa.ab, la, i, nd, m, acs, P.estimatedCoverage, qwi,
+                                          [&](int ii) { return base_at(L.aw, ii); }, f);
                         } else {
                             for (int k = 0; k < kMlpIn; k++) f[k] = 0.0f;
                         }
                         for (int k = 0; k < kMlpIn; k++) xt[tid * kMlpIn + k] = f2bf(f[k]);
-                        if (P.gateFeat && tid < ng) {
+                        if (P.gateFeat && !P.gateRecordCands && tid < ng) {
                             const int slot = atomicAdd(P.gateCount, 1);
                             if (slot < P.gateCap) {
                                 const int i = int(L.dec[g0 + tid]);
@@ -974,13 +1063,17 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             if (hq && P.correctCandidates) {
                 const int ae = msa.ab + la;
                 const bool forestC = P.correctionTypeCands == 1;
+                // learned gate on candidate columns (gated mode only): the
+                // forest's candidate range and decision rule, the MLP deciding
+                const bool gateC = kGateCands && P.correctionTypeCands == 3;
                 for (int c = wave; c < m; c += nwaves) {
                     const CandMeta cm = cur[c];
                     const int qb = msa.ab + cm.shift;
                     const int qe = qb + cm.len;
-                    const bool inRange = forestC ? (qb >= msa.ab - P.newColumns && qe <= ae + P.newColumns)
-                                                 : (msa.ab - P.newColumns <= qb && qb <= msa.ab + P.newColumns &&
-                                                    qe <= ae + P.newColumns);
+                    const bool inRange = (forestC || gateC)
+                                             ? (qb >= msa.ab - P.newColumns && qe <= ae + P.newColumns)
+                                             : (msa.ab - P.newColumns <= qb && qb <= msa.ab + P.newColumns &&
+                                                qe <= ae + P.newColumns);
                     if (!inRange) continue;
                     int slot = 0;
                     if (lane == 0) slot = atomicAdd(P.candOutCount, 1);
@@ -988,7 +1081,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     if (slot >= P.candOutCap) continue;
                     if (lane == 0) P.candOutIds[slot] = cm.id;
                     ColumnStats ccs{};
-                    if (forestC) {
+                    if (forestC || gateC) {
                         if (lane == 0) ccs = column_stats(L, qb, qe);
                         ccs.avgSupport = __shfl(ccs.avgSupport, 0, kWave);
                         ccs.minSupport = __shfl(ccs.minSupport, 0, kWave);
@@ -1000,6 +1093,12 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     char* dst = P.candOutSeq + u64(slot) * P.outPitch;
                     const bool rc = cm.flag == 2;
                     const u32* enc = rs.seqs + u64(cm.id) * rs.pitch;
+                    u8* cacc = nullptr;
+                    if (gateC) {
+                        char* ws = L.gscr + size_t(wave) * cand_gate_scratch(dims.maxLen);
+                        cacc = reinterpret_cast<u8*>(ws + kCandGateTile);
+                        gate_candidate(P, L, rs, msa.cols, qb, cm, m, ccs, ws, cacc);
+                    }
                     for (int i = lane; i < cm.len; i += kWave) {
                         // position i of the candidate in MSA orientation, written
                         // to its forward-strand position
@@ -1013,6 +1112,9 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                                 f[13] = jac;
                                 if (!forest_decide(P.forestC, f)) b = orig;
                             }
+                        } else if (gateC) {
+                            const u32 orig = oriented_base(enc, cm.len, i, rc);
+                            if (b != orig && !cacc[i]) b = orig;
                         }
                         const int out = rc ? cm.len - 1 - i : i;
                         dst[out] = "ACGT"[rc ? 3u - b : b];
@@ -1041,8 +1143,11 @@ static LdsDims make_dims(int maxLen, int useQuality, int qpitch, int threads, in
     d.chunk = chunk > 0 ? chunk : std::max(16, std::min(256, 16384 / rowBytes));
     d.gated = gated;
     if (gated) {
-        // the staging area doubles as the gate's feature tile and MLP scratch
-        const int need = 32 * kMlpIn * 2 + 32 * 4 + 2 * kMlpTileScratch + 32 * (kMlpIn + 1) * 4;
+        // the staging area doubles as the gates' feature tiles and MLP scratch
+        // (bit 0: anchor gate, two waves; bit 1: candidate gate, every wave)
+        int need = 0;
+        if (gated & 1) need = 32 * kMlpIn * 2 + 32 * 4 + 2 * kMlpTileScratch + 32 * (kMlpIn + 1) * 4;
+        if (gated & 2) need = std::max(need, d.waves * cand_gate_scratch(maxLen));
         while (d.chunk * rowBytes < need) d.chunk++;
     }
     return d;
@@ -1059,17 +1164,27 @@ int correct_msa_chunk(int maxLen, int useQuality, int qpitch, int threads, int g
 int correct_resident_blocks(int threads, size_t ldsBytes, int gated) {
     int nb = 0;
     const hipError_t e =
-        gated ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<true>, threads, ldsBytes)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<false>, threads, ldsBytes);
+        (gated & 2)   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<true, true>, threads, ldsBytes)
+        : (gated & 1) ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<true, false>, threads, ldsBytes)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<false, false>, threads, ldsBytes);
     return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
+int correct_gate_mode(int correctionType, int correctCandidates, int correctionTypeCands) {
+    return (correctionType == 3 ? 1 : 0) | (correctionType == 3 && correctCandidates && correctionTypeCands == 3 ? 2 : 0);
+}
+
 void launch_correct(const CorrectParams& p, int blocks, int threads, int chunk, hipStream_t stream) {
-    const LdsDims d = make_dims(p.rs.maxLen, p.useQuality, p.rs.qpitch, threads, p.correctionType == 3, chunk);
-    if (p.correctionType == 3)
-        hipLaunchKernelGGL(k_correct<true>, dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
+    const int gm = correct_gate_mode(p.correctionType, p.correctCandidates, p.correctionTypeCands);
+    const LdsDims d = make_dims(p.rs.maxLen, p.useQuality, p.rs.qpitch, threads, gm, chunk);
+    // the candidate gate has its own instantiation: its registers would lower
+    // the occupancy of the anchor-only gated mode
+    if (gm & 2)
+        hipLaunchKernelGGL((k_correct<true, true>), dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
+    else if (gm & 1)
+        hipLaunchKernelGGL((k_correct<true, false>), dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
     else
-        hipLaunchKernelGGL(k_correct<false>, dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
+        hipLaunchKernelGGL((k_correct<false, false>), dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
 }
 
 }  // namespace care_engine

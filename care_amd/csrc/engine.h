@@ -149,6 +149,14 @@ struct CorrectParams {
     u8* gateCons;
     int* gateCount;
     int gateCap;
+    int gateRecordCands;  // record candidate decision columns instead of the anchor's
+    // learned gate on candidate columns (correctionTypeCands 3 in the gated
+    // mode): weights, parameters and logit threshold
+    const unsigned short* gateCW1;
+    const unsigned short* gateCW2;
+    const unsigned short* gateCW3;
+    const float* gateCParams;
+    float gateCLogit;
 };
 
 // launchers (all asynchronous on `stream`)
@@ -167,7 +175,9 @@ void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, 
                   hipStream_t stream);
 // totals[0] += sum(rawCount[0..n)), totals[1] += sum(candCount[0..n))
 void launch_sum_counts(const int* rawCount, const int* candCount, int n, unsigned long long* totals, hipStream_t s);
-// chunk: candidates staged per MSA pass (0 = automatic)
+// chunk: candidates staged per MSA pass (0 = automatic); gated: bit 0 the
+// anchor gate (correction type 3), bit 1 the candidate gate
+int correct_gate_mode(int correctionType, int correctCandidates, int correctionTypeCands);
 void launch_correct(const CorrectParams& p, int blocks, int threads, int chunk, hipStream_t stream);
 size_t correct_lds_bytes(int maxLen, int useQuality, int qpitch, int threads, int gated, int chunk);
 int correct_msa_chunk(int maxLen, int useQuality, int qpitch, int threads, int gated, int chunk);

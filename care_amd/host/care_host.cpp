@@ -310,7 +310,8 @@ struct HostOptions {
     int batchsize = 0;
     bool autotune = true;  // time launch shapes on a sample of anchors before STEP 2
     std::string forestAnchor, forestCands;
-    std::string gateFile;  // learned gate parameters (correctionType 3)
+    std::string gateFile;       // learned gate parameters (correctionType 3)
+    std::string gateFileCands;  // candidate gate parameters (correctionTypeCands 3; default: the anchor gate)
     uint32_t maxTreesAnchor = 0xFFFFFFFFu, maxTreesCands = 0xFFFFFFFFu;
     bool haveCoverage = false, havePairmode = false;
 };
@@ -378,6 +379,7 @@ HostOptions parseOptions(int argc, char** argv) {
         else if (a == "--correctionTypeCands") o.eng.correction_type_cands = std::stoi(value());
         else if (a == "--ml-forestfile") o.forestAnchor = value();
         else if (a == "--ml-gatefile") o.gateFile = value();
+        else if (a == "--ml-cands-gatefile") o.gateFileCands = value();
         else if (a == "--candidateLimit") o.eng.candidate_limit = std::stoi(value());
         else if (a == "--ml-cands-forestfile") o.forestCands = value();
         else if (a == "--thresholdAnchor") o.eng.threshold_anchor = std::stof(value());
@@ -588,18 +590,21 @@ int runMain(int argc, char** argv) {
     std::unique_ptr<care_context, int (*)(care_context*)> guard(ctx, care_context_destroy);
     if (o.batchsize > 0) check(care_set_launch_config(ctx, (o.batchsize + 1) & ~1, 0));
     if (o.eng.correction_type == 1) check(care_forest_load(ctx, 0, o.forestAnchor.c_str(), o.maxTreesAnchor));
-    if (o.eng.correction_type == 3) {
-        // gate parameters: CARE_MLP_PARAMS little-endian float32 values
+    // gate parameters: CARE_MLP_PARAMS little-endian float32 values
+    auto readGate = [](const std::string& path, const char* flag) {
         std::vector<float> params(CARE_MLP_PARAMS);
-        FILE* f = std::fopen(o.gateFile.c_str(), "rb");
-        if (!f) throw std::runtime_error("cannot open gate file '" + o.gateFile + "' (--ml-gatefile)");
+        FILE* f = std::fopen(path.c_str(), "rb");
+        if (!f) throw std::runtime_error("cannot open gate file '" + path + "' (" + flag + ")");
         const size_t got = std::fread(params.data(), sizeof(float), params.size(), f);
         const bool extra = std::fgetc(f) != EOF;
         std::fclose(f);
         if (got != params.size() || extra)
-            throw std::runtime_error("gate file '" + o.gateFile + "' must hold exactly CARE_MLP_PARAMS float32 values");
-        check(care_mlp_load(ctx, params.data()));
-    }
+            throw std::runtime_error("gate file '" + path + "' must hold exactly CARE_MLP_PARAMS float32 values");
+        return params;
+    };
+    if (o.eng.correction_type == 3) check(care_mlp_load(ctx, readGate(o.gateFile, "--ml-gatefile").data()));
+    if (o.eng.correction_type_cands == 3 && !o.gateFileCands.empty())
+        check(care_mlp_load_candidates(ctx, readGate(o.gateFileCands, "--ml-cands-gatefile").data()));
     if (o.eng.correct_candidates && o.eng.correction_type_cands == 1)
         check(care_forest_load(ctx, 1, o.forestCands.c_str(), o.maxTreesCands));
     check(care_readstore_upload(ctx, rs->n, rs->maxLen, rs->seqs.data(), rs->pitch, rs->lens.data(), rs->ambig.data(),

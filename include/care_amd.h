@@ -46,7 +46,8 @@ typedef struct care_options {
     float coverage_factor;       /* --coveragefactortuning (m_coverage) */
     float paired_filter_threshold; /* --pairedFilterThreshold */
     int correction_type;         /* --correctionType: 0 classic, 1 forest, 3 learned gate */
-    int correction_type_cands;   /* --correctionTypeCands: 0 classic, 1 forest */
+    int correction_type_cands;   /* --correctionTypeCands: 0 classic, 1 forest, 3 learned gate
+                                    (with correction_type 3) */
     float threshold_anchor;      /* --thresholdAnchor */
     float threshold_cands;       /* --thresholdCands */
     int candidate_limit;         /* correction_type 3 only: keep the candidates hit by the
@@ -194,6 +195,9 @@ typedef struct care_mlp_train_options {
 
 /* Sets the gate used by care_correct_batch under correction_type 3. */
 int care_mlp_load(care_context* ctx, const float* params);
+/* Parameters of the gate on candidate columns (correction_type_cands 3);
+ * without them the anchor gate's parameters are used. */
+int care_mlp_load_candidates(care_context* ctx, const float* params);
 
 /* Gradient of the mean binary cross-entropy over n rows (features [n][32],
  * labels 0/1) at params, computed by the device's training kernels; loss_out
@@ -217,6 +221,16 @@ int care_mlp_infer(care_context* ctx, const float* features, uint64_t n, const f
  * columns; CARE_ERR_CAPACITY if they exceed capacity. */
 int care_gate_columns(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
                       uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count);
+
+/* Candidate counterpart of care_gate_columns: the decision columns of the
+ * candidate corrections (candidates of HQ anchors within new_columns of the
+ * anchor, positions where the consensus differs from the candidate) with
+ * their 32 features, read id, forward-strand position and forward-strand
+ * original and consensus bases.  Runs correction_type_cands 3 over the
+ * batch; needs correction_type 3. */
+int care_gate_columns_candidates(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
+                                 uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity,
+                                 uint64_t* count);
 
 /* Timing and work counters of the last care_correct_batch call. */
 typedef struct care_batch_stats {

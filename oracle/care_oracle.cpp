@@ -65,6 +65,8 @@ struct Options {
     int candidateLimit = 0;      // correctionType 3: top candidates by map hits
     int gateConstant = 1;        // correctionType 3: the oracle's gate accepts (1) or rejects (-1) every column
     std::string gateDump;        // correctionType 3: binary record of every decision column
+    int gateConstantCands = 1;   // correctionTypeCands 3: the same for candidate columns
+    std::string gateDumpCands;   // correctionTypeCands 3: binary record of every candidate decision column
     int correctionTypeCands = 0;
     float thresholdAnchor = 0.5f;
     float thresholdCands = 0.5f;
@@ -996,15 +998,19 @@ struct Corrector {
         std::vector<Result> cands;
         bool hq = false;
         std::vector<GateColumn> gate;
+        std::vector<GateColumn> gateCands;
     };
 
     static int baseCode(char c) { return c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : 0; }
 
-    // The engine's 32 gate features of anchor position i (no counterpart in
-    // the reference; restated from care_amd/csrc/correct.hip gate_features).
+    // The engine's 32 gate features of position i of a read in the MSA
+    // starting at column `begin` (the anchor, or a candidate in its MSA
+    // orientation with its orientation's qualities); p are the MSA
+    // properties over the read's columns (no counterpart in the reference;
+    // restated from care_amd/csrc/correct.hip gate_features).
     void gateFeatures(float* f, const MSA& msa, const MSA::Props& p, const std::string& anchor, const std::string& anchorQ,
-                      int i, int nd, int m) const {
-        const int col = msa.anchorBegin + i;
+                      int begin, int i, int nd, int m) const {
+        const int col = begin + i;
         const int cols = msa.nColumns;
         const int la = int(anchor.size());
         const float estCov = opt.estimatedCoverage;
@@ -1013,7 +1019,7 @@ struct Corrector {
         const float wsum = msa.wA[col] + msa.wC[col] + msa.wG[col] + msa.wT[col];
         const float wo = weightOf(msa, o, col), wc = weightOf(msa, c, col);
         const float no = countOf(msa, o, col), nc = countOf(msa, c, col);
-        auto mism = [&](int ii) { return msa.consensus[msa.anchorBegin + ii] != anchor[ii]; };
+        auto mism = [&](int ii) { return msa.consensus[begin + ii] != anchor[ii]; };
         f[0] = no / cov;
         f[1] = wo / wsum;
         f[2] = nc / cov;
@@ -1218,7 +1224,7 @@ struct Corrector {
                     g.pos = std::uint16_t(i);
                     g.orig = std::uint8_t(baseCode(anchor[i]));
                     g.cons = std::uint8_t(baseCode(msa.consensus[msa.anchorBegin + i]));
-                    gateFeatures(g.f, msa, props, anchor, anchorQ, i, int(dec.size()), int(cands.size()));
+                    gateFeatures(g.f, msa, props, anchor, anchorQ, msa.anchorBegin, i, int(dec.size()), int(cands.size()));
                     out.gate.push_back(g);
                     if (opt.gateConstant < 0) corrected[i] = anchor[i];
                 }
@@ -1256,9 +1262,12 @@ struct Corrector {
                 const int qe = qb + len;
                 const int ncc = opt.new_columns_to_correct;
                 const bool forest = opt.correctionTypeCands == 1;
-                const bool inRange = forest ? (qb >= msa.anchorBegin - ncc && qe <= msa.anchorEnd + ncc)
-                                            : (msa.anchorBegin - ncc <= qb && qb <= msa.anchorBegin + ncc &&
-                                               qe <= msa.anchorEnd + ncc);
+                // the engine's learned gate on candidate columns (gated mode):
+                // the forest's range and rule, the gate deciding
+                const bool gateC = opt.correctionType == 3 && opt.correctionTypeCands == 3;
+                const bool inRange = (forest || gateC) ? (qb >= msa.anchorBegin - ncc && qe <= msa.anchorEnd + ncc)
+                                                       : (msa.anchorBegin - ncc <= qb && qb <= msa.anchorBegin + ncc &&
+                                                          qe <= msa.anchorEnd + ncc);
                 if (!inRange) continue;
                 std::string seq(msa.consensus.begin() + qb, msa.consensus.begin() + qe);
                 if (forest) {
@@ -1269,6 +1278,26 @@ struct Corrector {
                             candFeatures(f, msa, cprops, cands[c].chars[i], qb, qe, i);
                             if (!forestC->decide(f)) seq[i] = cands[c].chars[i];
                         }
+                    }
+                } else if (gateC) {
+                    const MSA::Props cprops = msa.properties(qb, qe, opt.estimatedErrorrate, opt.estimatedCoverage, opt.m_coverage);
+                    std::vector<int> dec;
+                    for (int i = 0; i < len; i++)
+                        if (cands[c].chars[i] != msa.consensus[qb + i]) dec.push_back(i);
+                    const std::string noQ(len, '\0');
+                    const bool rcC = cands[c].flag == 2;
+                    for (int i : dec) {
+                        GateColumn g;
+                        g.id = cands[c].id;
+                        // forward-strand position and bases of the candidate
+                        g.pos = std::uint16_t(rcC ? len - 1 - i : i);
+                        const int o = baseCode(cands[c].chars[i]), cn = baseCode(msa.consensus[qb + i]);
+                        g.orig = std::uint8_t(rcC ? 3 - o : o);
+                        g.cons = std::uint8_t(rcC ? 3 - cn : cn);
+                        gateFeatures(g.f, msa, cprops, cands[c].chars, opt.useQualityScores ? cands[c].qual : noQ, qb, i,
+                                     int(dec.size()), int(cands.size()));
+                        out.gateCands.push_back(g);
+                        if (opt.gateConstantCands < 0) seq[i] = cands[c].chars[i];
                     }
                 }
                 Result r;
@@ -1509,6 +1538,8 @@ Options parseArgs(int argc, char** argv) {
         else if (a == "--candidateLimit") o.candidateLimit = std::stoi(next());
         else if (a == "--gateConstant") o.gateConstant = std::stoi(next());
         else if (a == "--gateDump") o.gateDump = next();
+        else if (a == "--gateConstantCands") o.gateConstantCands = std::stoi(next());
+        else if (a == "--gateDumpCands") o.gateDumpCands = next();
         else if (a == "--ml-forestfile") o.forestAnchor = next();
         else if (a == "--ml-cands-forestfile") o.forestCands = next();
         else if (a == "--thresholdAnchor") o.thresholdAnchor = std::stof(next());
@@ -1579,19 +1610,21 @@ int runMain(int argc, char** argv) {
     std::vector<AnchorDump> dumps;
     corr.run(numAnchors, outs, opt.dumpFile.empty() ? nullptr : &dumps);
     const double t2 = now();
-    if (!opt.gateDump.empty()) {
-        // per decision column: u32 read id, u16 position, u8 original and
-        // consensus base, 32 float features; anchors in order
-        std::ofstream g(opt.gateDump, std::ios::binary);
+    // per decision column: u32 read id, u16 position, u8 original and
+    // consensus base, 32 float features; anchors in order
+    auto dumpGate = [&](const std::string& path, bool cands) {
+        std::ofstream g(path, std::ios::binary);
         for (std::size_t a = 0; a < numAnchors; a++)
-            for (const auto& c : outs[a].gate) {
+            for (const auto& c : cands ? outs[a].gateCands : outs[a].gate) {
                 g.write(reinterpret_cast<const char*>(&c.id), 4);
                 g.write(reinterpret_cast<const char*>(&c.pos), 2);
                 g.write(reinterpret_cast<const char*>(&c.orig), 1);
                 g.write(reinterpret_cast<const char*>(&c.cons), 1);
                 g.write(reinterpret_cast<const char*>(c.f), sizeof(c.f));
             }
-    }
+    };
+    if (!opt.gateDump.empty()) dumpGate(opt.gateDump, false);
+    if (!opt.gateDumpCands.empty()) dumpGate(opt.gateDumpCands, true);
 
     // results per read; HQ anchors with zero edits are not stored and
     // candidate results of reads corrected as HQ anchors are dropped

@@ -30,6 +30,13 @@ SMALL = {
 }
 GPU_CASE = {"profile": "repeat-pe", "seed": 303, "genome": 40000,
             "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "2", "-k", "20", "-h", "32"]}
+# the gate on candidate corrections (correctionTypeCands 3)
+CAND_SMALL = {"profile": "repeat-pe", "seed": 304, "genome": 2000,
+              "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "2", "-k", "20", "-h", "32",
+                       "--candidateCorrection", "--correctionTypeCands", "3"]}
+CAND_GPU = {"profile": "large-pe", "seed": 305, "genome": 40000,
+            "args": ["-c", "40", "--pairmode", "PE", "-q", "-k", "20", "-h", "48",
+                     "--candidateCorrection", "--correctionTypeCands", "3"]}
 LIMIT = 12
 
 REC = np.dtype([("id", "<u4"), ("pos", "<u2"), ("orig", "u1"), ("cons", "u1"), ("f", "<f4", (32,))])
@@ -42,32 +49,39 @@ def constant_gate(path, accept):
     return p
 
 
-def _oracle(case, d, gate, dump=None):
+def _oracle(case, d, gate, dump=None, gate_cands=1, dump_cands=None):
     reads, _ = helpers.simulate(case, d)
-    out = os.path.join(d, "oracle%d" % gate)
+    out = os.path.join(d, "oracle%d_%d" % (gate, gate_cands))
     args = [helpers.ORACLE_EXE, "-i", reads, "-o", "r.fastq", "-d", out, "-t", "4", "--correctionType", "3",
-            "--candidateLimit", str(LIMIT), "--gateConstant", str(gate)] + case["args"]
+            "--candidateLimit", str(LIMIT), "--gateConstant", str(gate), "--gateConstantCands", str(gate_cands)] + case["args"]
     if dump:
         args += ["--gateDump", dump]
+    if dump_cands:
+        args += ["--gateDumpCands", dump_cands]
     p = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert p.returncode == 0, p.stdout[-2000:]
     return helpers.md5(os.path.join(out, "r.fastq"))
 
 
-def _engine_run(exe, case, d, gate):
+def _engine_run(exe, case, d, gate, gate_cands=None):
     reads, _ = helpers.simulate(case, d)
     gf = os.path.join(d, "gate%d.bin" % gate)
     constant_gate(gf, gate > 0)
-    out = os.path.join(d, "eng%d" % gate)
+    extra = []
+    if gate_cands is not None:
+        gfc = os.path.join(d, "gatec%d.bin" % gate_cands)
+        constant_gate(gfc, gate_cands > 0)
+        extra = ["--ml-cands-gatefile", gfc]
+    out = os.path.join(d, "eng%d_%s" % (gate, gate_cands))
     p = subprocess.run([exe, "-i", reads, "-o", "r.fastq", "-d", out, "-g", "0", "-t", "4", "--correctionType", "3",
-                        "--candidateLimit", str(LIMIT), "--ml-gatefile", gf] + case["args"],
+                        "--candidateLimit", str(LIMIT), "--ml-gatefile", gf] + extra + case["args"],
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=900)
     assert p.returncode == 0, p.stdout[-2000:]
     return helpers.md5(os.path.join(out, "r.fastq"))
 
 
-def _records(eng, rs):
-    f, ids, pos, orig, cons = eng.gate_columns(0, rs.n)
+def _records(eng, rs, candidates=False):
+    f, ids, pos, orig, cons = eng.gate_columns(0, rs.n, candidates=candidates)
     rec = np.zeros(len(ids), dtype=REC)
     rec["id"], rec["pos"], rec["orig"], rec["cons"], rec["f"] = ids, pos, orig, cons, f
     return np.sort(rec, order=["id", "pos"])
@@ -78,6 +92,8 @@ def _options(case):
     kw = dict(kmer_length=int(a[a.index("-k") + 1]), num_hash_maps=int(a[a.index("-h") + 1]),
               coverage=float(a[a.index("-c") + 1]), paired_end=int("PE" in a), use_quality=int("-q" in a),
               correction_type=3, candidate_limit=LIMIT)
+    if "--candidateCorrection" in a:
+        kw["correct_candidates"] = 1
     return kw
 
 
@@ -86,6 +102,27 @@ def _readset(case, d):
     a = case["args"]
     qb = int(a[a.index("--qualityScoreBits") + 1]) if "--qualityScoreBits" in a else (8 if "-q" in a else 0)
     return care_amd.ReadSet([reads], paired_end="PE" in a, qual_bits=qb)
+
+
+def _cand_records(lib, case, d):
+    """Candidate decision columns of the engine (with a constant accepting
+    gate) and of the oracle, sorted by read, position and features."""
+    dump = os.path.join(d, "gate_cands.bin")
+    _oracle(case, d, 1, gate_cands=1, dump_cands=dump)
+    want = np.fromfile(dump, dtype=REC)
+    rs = _readset(case, d)
+    eng = care_amd.Engine(care_amd.default_options(**_options(case)), device=0, lib=lib)
+    eng.upload_readset(rs)
+    eng.build()
+    eng.mlp_load(constant_gate(os.path.join(d, "g1.bin"), True))
+    f, ids, pos, orig, cons = eng.gate_columns(0, rs.n, candidates=True)
+    eng.close()
+    got = np.zeros(len(ids), dtype=REC)
+    got["id"], got["pos"], got["orig"], got["cons"], got["f"] = ids, pos, orig, cons, f
+    # a read can be corrected as the candidate of several anchors: compare
+    # the multisets of records
+    key = lambda r: np.lexsort([r["f"][:, k] for k in range(31, -1, -1)] + [r["cons"], r["orig"], r["pos"], r["id"]])
+    return got[key(got)], want[key(want)]
 
 
 def test_limited_query_oracle_bruteforce():
@@ -127,6 +164,9 @@ def test_limited_query_oracle_bruteforce():
 @pytest.fixture(scope="module")
 def emu():
     subprocess.run(["sh", os.path.join(EMU_DIR, "build_emu.sh")], check=True)
+    # device memory and LDS filled with a non-zero byte: features read from
+    # MSA columns nothing computed would differ from the oracle's
+    os.environ.setdefault("EMU_POISON", "0xA5")
     return care_amd._bind(ctypes.CDLL(EMU_LIB), care_amd.ENGINE_SYMBOLS)
 
 
@@ -152,6 +192,31 @@ def test_emulated_gate_columns_match_oracle(emu, name, tmp_path):
     if name == "pe_q":
         assert len(want) > 20
     assert np.array_equal(got, want)
+
+
+def _assert_same_records(got, want):
+    assert len(got) == len(want), (len(got), len(want))
+    bad = np.nonzero(got != want)[0]
+    if len(bad):
+        i = bad[0]
+        fd = np.nonzero(got[i]["f"] != want[i]["f"])[0]
+        raise AssertionError("%d differing records; first %d: engine %s / oracle %s, features %s: %s vs %s" % (
+            len(bad), i, got[i][["id", "pos", "orig", "cons"]], want[i][["id", "pos", "orig", "cons"]], fd.tolist(),
+            got[i]["f"][fd].tolist(), want[i]["f"][fd].tolist()))
+
+
+@pytest.mark.parametrize("gates", [(1, 1), (1, -1), (-1, -1)])
+def test_emulated_candidate_gate_matches_oracle(emu, gates, tmp_path):
+    """Candidate corrections through the candidate gate (accept / reject every
+    candidate decision column) equal the oracle's restatement."""
+    ga, gc = gates
+    assert _engine_run(EMU_EXE, CAND_SMALL, str(tmp_path), ga, gc) == _oracle(CAND_SMALL, str(tmp_path), ga, gate_cands=gc)
+
+
+def test_emulated_candidate_gate_columns_match_oracle(emu, tmp_path):
+    got, want = _cand_records(emu, CAND_SMALL, str(tmp_path))
+    assert len(want) > 20
+    _assert_same_records(got, want)
 
 
 # ------------------------------------------------------------------- GPU
@@ -200,3 +265,20 @@ def test_gpu_limited_candidate_lists_match_oracle(tmp_path):
             bad += 1
     orc.orc_tables_free(t)
     assert bad == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gates", [(1, 1), (1, -1)])
+def test_gpu_candidate_gate_matches_oracle(gates, tmp_path):
+    print("candidate gate, constant gates %s" % (gates,), flush=True)
+    ga, gc = gates
+    assert _engine_run(care_amd.CARE_GPU, CAND_GPU, str(tmp_path), ga, gc) == _oracle(CAND_GPU, str(tmp_path), ga,
+                                                                                       gate_cands=gc)
+
+
+@pytest.mark.gpu
+def test_gpu_candidate_gate_columns_match_oracle(tmp_path):
+    print("candidate gate columns", flush=True)
+    got, want = _cand_records(None, CAND_GPU, str(tmp_path))
+    assert len(want) > 100
+    _assert_same_records(got, want)
