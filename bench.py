@@ -253,6 +253,17 @@ def main():
                                                 threshold_anchor=threshold))
         gated = time_path(care_amd, rs, gopt, args.steps, args.warmup, gate=gate_mod.load(gate_path),
                           batch=args.batch, autotune=not args.no_autotune)
+    gated_large = {}
+    if args.workload == "large-pe" and os.path.exists(gate_path):
+        # the gate on anchors, and on anchors and candidate corrections
+        threshold = gate_mod.chosen_threshold(gate_path)
+        for name, extra in (("mlp_gated_anchors", {}),
+                            ("mlp_gated_anchors_and_candidates", dict(correction_type_cands=3, threshold_cands=0.99))):
+            gopt = care_amd.default_options(**dict(w["options"], correction_type=3,
+                                                    candidate_limit=gate_mod.CANDIDATE_LIMIT,
+                                                    threshold_anchor=threshold, **extra))
+            gated_large[name] = time_path(care_amd, rs, gopt, args.steps, args.warmup, gate=gate_mod.load(gate_path),
+                                          batch=args.batch, autotune=not args.no_autotune)
     rs.close()
 
     qual = None
@@ -311,6 +322,8 @@ def main():
                                    method="top-%d candidates by map hits, no refinement, MLP gate on non-HQ "
                                           "decision columns" % gate_mod.CANDIDATE_LIMIT)
         config["speedup_gated_vs_plain"] = round(plain["ms"] / gated["ms"], 3)
+    for name, r in gated_large.items():
+        config[name] = summary(r)
     if qual is not None:
         config["quality"] = qual
     line = {

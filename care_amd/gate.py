@@ -11,6 +11,7 @@ the device (care_mlp_train: BF16 MFMA forward/backward, Adam).
     python -m care_amd.gate evaluate --gate care_amd/gate_repeat_pe.bin
 """
 import argparse
+import re
 import json
 import os
 import subprocess
@@ -248,6 +249,47 @@ def evaluate(args):
     return res
 
 
+LARGE_ARGS = ["-c", "40", "--pairmode", "PE", "-q", "-k", "20", "-h", "48", "--candidateCorrection"]
+
+
+def evaluate_large(args):
+    """The largest configuration's generator (large-pe: 5 % repeats at 2 %
+    divergence, 2x150 bp PE at 40x, 8-bit qualities, h = 48, candidate
+    correction) at a reduced genome size: base-level TP/FP/FN and STEP 2
+    time of the command line for the plain path, the gate on anchors and the
+    gate on anchors and candidate corrections (same network and threshold,
+    candidate thresholds as given)."""
+    work = args.work or tempfile.mkdtemp(prefix="care_gate_large_")
+    os.makedirs(work, exist_ok=True)
+    reads = os.path.join(work, "large.fastq")
+    truth = reads + ".truth"
+    if not os.path.exists(reads):
+        care_amd.simulate("large-pe", args.seed, reads, truth, genome=args.genome)
+    gate_path = os.path.abspath(args.gate)
+    t = chosen_threshold(gate_path)
+
+    def run(name, extra):
+        outdir = os.path.join(work, name)
+        cmd = [care_amd.CARE_GPU, "-i", reads, "-o", "c.fastq", "-d", outdir, "-g", "0", "-t", "16"] + LARGE_ARGS + extra
+        p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if p.returncode != 0:
+            raise RuntimeError(p.stdout[-2000:])
+        m = re.search(r"elapsed time \(STEP2\): ([0-9.]+)s", p.stdout)
+        r = care_amd.evaluate(reads, os.path.join(outdir, "c.fastq"), truth)
+        r = {k: r[k] for k in ("TP", "FP", "FN")}
+        r["step2_s"] = float(m.group(1)) if m else None
+        print("gate large: %s %s" % (name, json.dumps(r)), flush=True)
+        return r
+
+    res = {"reads": None, "plain": run("plain", []), "gated_anchors": run("gated_anchors", _gated_args(gate_path, t)),
+           "gated_both": {}}
+    for tc in args.thresholds_cands:
+        res["gated_both"][repr(tc)] = run("gated_both_%g" % tc, _gated_args(gate_path, t) + [
+            "--correctionTypeCands", "3", "--thresholdCands", repr(tc)])
+    print(json.dumps(res), flush=True)
+    return res
+
+
 def timing(rows, epochs=1):
     """Device training and inference throughput of the gate on synthetic
     columns (a fixed linear rule), for kernel profiling."""
@@ -291,11 +333,19 @@ def main(argv=None):
     e.add_argument("--work", default=None)
     e.add_argument("--thresholds", type=lambda v: [float(x) for x in v.split(",")], default=None,
                    help="acceptance thresholds to evaluate (default: the one chosen at training)")
+    el = sub.add_parser("evaluate-large")
+    el.add_argument("--gate", default=DEFAULT_GATE)
+    el.add_argument("--seed", type=int, default=20240605)
+    el.add_argument("--genome", type=int, default=10_000_000)
+    el.add_argument("--work", default=None)
+    el.add_argument("--thresholds-cands", type=lambda v: [float(x) for x in v.split(",")], default=[0.5, 0.9, 0.99])
     a = ap.parse_args(argv)
     if a.cmd == "train":
         train(a)
     elif a.cmd == "timing":
         timing(a.rows, a.epochs)
+    elif a.cmd == "evaluate-large":
+        evaluate_large(a)
     elif a.cmd == "select":
         select(a.gate, a.work or tempfile.mkdtemp(prefix="care_gate_sel_"))
     else:
