@@ -330,8 +330,9 @@ int care_context_destroy(care_context* ctx) {
 
 int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threads) {
     if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
-    if (batch_anchors < 0 || (block_threads != 0 && block_threads != 128 && block_threads != 256))
-        return fail(CARE_ERR_ARGUMENT, "invalid launch configuration (block_threads 128 or 256)");
+    if (batch_anchors < 0 ||
+        (block_threads != 0 && block_threads != 128 && block_threads != 192 && block_threads != 256))
+        return fail(CARE_ERR_ARGUMENT, "invalid launch configuration (block_threads 128, 192 or 256)");
     if (batch_anchors > 0 && (batch_anchors & 1)) return fail(CARE_ERR_ARGUMENT, "batch_anchors must be even");
     ctx->requestedBatch = batch_anchors;
     if (block_threads) ctx->blockThreads = block_threads;
@@ -341,8 +342,8 @@ int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threa
 int care_set_tuning(care_context* ctx, const care_tuning* t) {
     if (!ctx || !t) return fail(CARE_ERR_ARGUMENT, "null argument");
     if (t->batch_anchors < 0 || (t->batch_anchors & 1)) return fail(CARE_ERR_ARGUMENT, "batch_anchors must be even and >= 0");
-    if (t->correct_threads != 0 && t->correct_threads != 128 && t->correct_threads != 256)
-        return fail(CARE_ERR_ARGUMENT, "correct_threads must be 128 or 256");
+    if (t->correct_threads != 0 && t->correct_threads != 128 && t->correct_threads != 192 && t->correct_threads != 256)
+        return fail(CARE_ERR_ARGUMENT, "correct_threads must be 128, 192 or 256");
     if (t->correct_blocks_per_cu < 0 || t->correct_blocks_per_cu > 64)
         return fail(CARE_ERR_ARGUMENT, "correct_blocks_per_cu outside 0..64");
     if (t->msa_chunk != 0 && (t->msa_chunk < 16 || t->msa_chunk > 256))
@@ -1342,7 +1343,7 @@ int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen)
         ctx->queryCap = bestCap;
         query(bestCap);
         HIP_TRY(hipGetLastError());
-        // correction: 2 or 4 wavefronts per anchor, automatic and halved MSA chunks
+        // correction: 2, 3 or 4 wavefronts per anchor, automatic and halved MSA chunks
         P.candIds = ctx->candIds.p;
         P.candCount = ctx->candCount.p;
         P.metaA = ctx->metaA.p;
@@ -1360,7 +1361,7 @@ int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen)
         const int gatedK = correct_gate_mode(o.correction_type, o.correct_candidates, o.correction_type_cands);
         int bestT = ctx->blockThreads, bestChunk = ctx->msaChunk;
         float bestC = 1e30f;
-        for (int T : {256, 128}) {
+        for (int T : {256, 192, 128}) {
             const int autoChunk = correct_msa_chunk(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T, gatedK, 0);
             for (int chunk : {0, autoChunk / 2}) {
                 if (chunk != 0 && (chunk < 16 || chunk == autoChunk)) continue;

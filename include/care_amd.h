@@ -127,7 +127,7 @@ typedef struct care_batch_result {
 int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* result);
 
 /* Number of anchors per internal launch (even, 0 = automatic) and threads
- * per correction workgroup (128 or 256, 0 = keep); the other launch shapes
+ * per correction workgroup (128, 192 or 256, 0 = keep); the other launch shapes
  * stay as set by care_set_tuning / care_autotune.  The reference fixes these
  * at compile time (e.g. blocksize 128 in src/gpu/alignmentkernels.cu:652,
  * src/gpu/msakernels.cu:1186). */
@@ -140,7 +140,7 @@ int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threa
 typedef struct care_tuning {
     int batch_anchors;          /* anchors per internal launch, even (0 = automatic) */
     int correct_threads;        /* threads per correction workgroup, i.e. 2 or 4 wavefronts
-                                   per anchor: 128 or 256 (0 = 256) */
+                                   per anchor: 128, 192 or 256 (0 = 256) */
     int correct_blocks_per_cu;  /* persistent correction workgroups per compute unit
                                    (0 = as many as are resident at once) */
     int msa_chunk;              /* candidates staged in LDS per MSA pass, 16..256
