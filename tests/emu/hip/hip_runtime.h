@@ -456,6 +456,8 @@ inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
     return hipSuccess;
 }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+// launches and copies complete when issued, so cross-stream waits are no-ops
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventCreate(hipEvent_t* e) {
     *e = new emu_event{0};

@@ -2,8 +2,9 @@
 
 The query and correction kernels are run under the default shapes, under
 shapes that keep every hit list in global scratch instead of LDS (LDS
-capacity 16 hits) with two wavefronts per anchor and the smallest MSA
-chunk, under shapes that merge hit lists of more than 64 entries list by
+capacity 16 hits) with two wavefronts per anchor, the smallest MSA chunk and
+512-anchor device batches (each batch's query overlapping the previous
+batch's correction), under shapes that merge hit lists of more than 64 entries list by
 list instead of sorting them in registers (in LDS and in global scratch),
 and under the shapes the autotuner keeps; candidate lists, anchor status
 and corrected anchors must be identical, and the candidate lists equal the
@@ -36,7 +37,9 @@ GPU = {"profile": "repeat-pe", "seed": 213, "genome": 40000,
        "options": dict(kmer_length=20, num_hash_maps=32, coverage=30.0, paired_end=1, use_quality=1),
        "qual_bits": 2}
 
-SPILL = dict(query_lds_hits=16, correct_threads=128, msa_chunk=16)
+# 512-anchor batches: many device batches, the query of each overlapping the
+# correction of the previous one (two buffer sets, two streams)
+SPILL = dict(query_lds_hits=16, correct_threads=128, msa_chunk=16, batch_anchors=512)
 # lists longer than 64 hits merged list by list instead of sorted in registers,
 # three wavefronts per anchor
 MERGE = dict(query_register_hits=64, correct_threads=192)
@@ -94,6 +97,7 @@ def _check(lib, case, d, device_autotune):
         if tuning == SPILL:
             got = eng.tuning()
             assert got["query_lds_hits"] == 16 and got["correct_threads"] == 128 and got["msa_chunk"] == 16
+            assert got["batch_anchors"] == 512 and eng.stats().launches == (rs.n + 511) // 512
         if chosen is not None:
             assert chosen["correct_threads"] in (128, 192, 256)
             assert chosen["query_lds_hits"] % 16 == 0 and chosen["correct_blocks_per_cu"] >= 1
