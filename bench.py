@@ -288,7 +288,7 @@ def main():
         alg = n * (pitch_bytes + w["max_length"] + 1) + r["candidates"] * (pitch_bytes + 4 + 16 + qbytes)
         launch_ms = r["correct_ms"] / r["launches"]
         achieved = (alg / r["launches"]) / (launch_ms / 1e3) / 1e9
-        kernel = "k_correct<%s>" % ("true" if r is gated else "false")
+        kernel = "k_correct<%s,false>" % ("true" if r is gated else "false")
         return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kernel, args.workload),
                 "kernel": kernel, "launches_per_step": r["launches"],

@@ -3,9 +3,9 @@
 The query and correction kernels are run under the default shapes, under
 shapes that keep every hit list in global scratch instead of LDS (LDS
 capacity 16 hits) with two wavefronts per anchor, the smallest MSA chunk and
-512-anchor device batches (each batch's query overlapping the previous
-batch's correction), under shapes that merge hit lists of more than 64 entries list by
-list instead of sorting them in registers (in LDS and in global scratch),
+512-anchor device batches, under shapes that merge hit lists of more than 64
+entries list by list instead of sorting them in registers (in LDS and in
+global scratch),
 and under the shapes the autotuner keeps; candidate lists, anchor status
 and corrected anchors must be identical, and the candidate lists equal the
 oracle's.  The CPU tests drive the kernels through the HIP
@@ -37,8 +37,8 @@ GPU = {"profile": "repeat-pe", "seed": 213, "genome": 40000,
        "options": dict(kmer_length=20, num_hash_maps=32, coverage=30.0, paired_end=1, use_quality=1),
        "qual_bits": 2}
 
-# 512-anchor batches: many device batches, the query of each overlapping the
-# correction of the previous one (two buffer sets, two streams)
+# 512-anchor batches: many device batches enqueued back to back, candidate
+# results appended across batches
 SPILL = dict(query_lds_hits=16, correct_threads=128, msa_chunk=16, batch_anchors=512)
 # lists longer than 64 hits merged list by list instead of sorted in registers,
 # three wavefronts per anchor
