@@ -117,6 +117,10 @@ struct care_context {
     care_options opt{};
     hipStream_t stream = nullptr;
     hipEvent_t ev[4]{};
+    // device-to-host copies of each device batch's results, overlapping the
+    // next batches' kernels; copyDone joins it back into `stream`
+    hipStream_t copyStream = nullptr;
+    hipEvent_t copyDone = nullptr;
     int numCUs = 0;
 
     // read store
@@ -287,6 +291,8 @@ int care_context_create(int device, const care_options* opt, care_context** out)
         ctx->opt = *opt;
         HIP_TRY(hipSetDevice(device));
         HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->copyStream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&ctx->copyDone, hipEventDisableTiming));
         for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipDeviceGetAttribute(&ctx->numCUs, hipDeviceAttributeMultiprocessorCount, device));
         ctx->cap = resultsPerMapThreshold(opt->coverage);
@@ -320,9 +326,12 @@ int care_context_destroy(care_context* ctx) {
     if (!ctx) return CARE_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copyStream) (void)hipStreamSynchronize(ctx->copyStream);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->batchEv) (void)hipEventDestroy(e);
+    if (ctx->copyDone) (void)hipEventDestroy(ctx->copyDone);
+    if (ctx->copyStream) (void)hipStreamDestroy(ctx->copyStream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CARE_OK;
@@ -614,6 +623,18 @@ void copy_rows_to_host(char* dst, size_t dpitch, const char* src, size_t spitch,
         HIP_TRY(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyDeviceToHost, s));
     }
 }
+
+// Page-locked host memory (hipHostMalloc): an asynchronous copy into it runs
+// on a copy engine without blocking the host; a copy into pageable memory
+// blocks the host until the copy is done.
+bool is_page_locked(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory may be reported as an error
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
 }
 
 extern "C" int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
@@ -760,6 +781,11 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
             HIP_TRY(hipEventCreate(&e));
             ctx->batchEv.push_back(e);
         }
+        // into page-locked destinations each batch's anchor results are copied
+        // as soon as the batch is done, overlapping the next batches; into
+        // pageable ones once at the end (a pageable copy would stall the
+        // host between batches)
+        const bool overlapCopies = is_page_locked(r->anchor_status) && is_page_locked(r->anchor_sequences);
         launch_fill_u64(reinterpret_cast<u64*>(ctx->rawHits.p), 2, 0, s);  // totals over the call
         launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);  // candidate results over the call
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
@@ -784,10 +810,24 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipEventRecord(e[2], s));
             ctx->stats.launches++;
+            if (overlapCopies) {
+                hipStream_t cs = ctx->copyStream;
+                HIP_TRY(hipStreamWaitEvent(cs, e[2], 0));
+                HIP_TRY(hipMemcpyAsync(r->anchor_status + done, ctx->outStatus.p + done, size_t(nb),
+                                       hipMemcpyDeviceToHost, cs));
+                copy_rows_to_host(r->anchor_sequences + size_t(done) * size_t(r->sequence_pitch),
+                                  size_t(r->sequence_pitch), ctx->outSeq.p + size_t(done) * size_t(ctx->outPitch),
+                                  size_t(ctx->outPitch), size_t(ctx->rs.maxLen), size_t(nb), cs);
+            }
         }
-        HIP_TRY(hipMemcpyAsync(r->anchor_status, ctx->outStatus.p, size_t(n), hipMemcpyDeviceToHost, s));
-        copy_rows_to_host(r->anchor_sequences, size_t(r->sequence_pitch), ctx->outSeq.p, size_t(ctx->outPitch),
-                          size_t(ctx->rs.maxLen), size_t(n), s);
+        if (overlapCopies) {
+            HIP_TRY(hipEventRecord(ctx->copyDone, ctx->copyStream));
+            HIP_TRY(hipStreamWaitEvent(s, ctx->copyDone, 0));
+        } else {
+            HIP_TRY(hipMemcpyAsync(r->anchor_status, ctx->outStatus.p, size_t(n), hipMemcpyDeviceToHost, s));
+            copy_rows_to_host(r->anchor_sequences, size_t(r->sequence_pitch), ctx->outSeq.p, size_t(ctx->outPitch),
+                              size_t(ctx->rs.maxLen), size_t(n), s);
+        }
         if (o.correct_candidates) {
             HIP_TRY(hipMemcpyAsync(ctx->hInts.p, ctx->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));

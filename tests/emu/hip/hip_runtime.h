@@ -23,7 +23,9 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -52,7 +54,7 @@ typedef void* hipStream_t;
 typedef struct emu_event* hipEvent_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1 };
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
-enum { hipStreamNonBlocking = 1, hipHostMallocDefault = 0 };
+enum { hipStreamNonBlocking = 1, hipHostMallocDefault = 0, hipEventDisableTiming = 2 };
 enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
 
 namespace emu {
@@ -463,6 +465,7 @@ inline hipError_t hipEventCreate(hipEvent_t* e) {
     *e = new emu_event{0};
     return hipSuccess;
 }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 inline hipError_t hipEventDestroy(hipEvent_t e) {
     delete e;
     return hipSuccess;
@@ -497,12 +500,53 @@ inline hipError_t hipFree(void* p) {
     std::free(p);
     return hipSuccess;
 }
+// page-locked allocations are remembered so that pointer queries can tell
+// them from pageable memory
+namespace emu {
+inline std::mutex& host_alloc_mutex() {
+    static std::mutex m;
+    return m;
+}
+inline std::map<const char*, size_t>& host_allocs() {
+    static std::map<const char*, size_t> m;
+    return m;
+}
+}  // namespace emu
 inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
     *p = std::calloc(1, bytes ? bytes : 1);
-    return *p ? hipSuccess : hipErrorInvalidValue;
+    if (!*p) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> g(emu::host_alloc_mutex());
+    emu::host_allocs()[static_cast<const char*>(*p)] = bytes ? bytes : 1;
+    return hipSuccess;
 }
 inline hipError_t hipHostFree(void* p) {
+    {
+        std::lock_guard<std::mutex> g(emu::host_alloc_mutex());
+        emu::host_allocs().erase(static_cast<const char*>(p));
+    }
     std::free(p);
+    return hipSuccess;
+}
+enum hipMemoryType { hipMemoryTypeUnregistered = 0, hipMemoryTypeHost = 1, hipMemoryTypeDevice = 2 };
+struct hipPointerAttribute_t {
+    hipMemoryType type;
+    int device;
+    void* devicePointer;
+    void* hostPointer;
+    int isManaged;
+    unsigned allocationFlags;
+};
+// page-locked memory from hipHostMalloc is reported as host memory, anything
+// else as an error (as for pageable memory on the device runtime)
+inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t* a, const void* p) {
+    std::lock_guard<std::mutex> g(emu::host_alloc_mutex());
+    const char* c = static_cast<const char*>(p);
+    auto& m = emu::host_allocs();
+    auto it = m.upper_bound(c);
+    if (it == m.begin()) return hipErrorInvalidValue;
+    --it;
+    if (c >= it->first + it->second) return hipErrorInvalidValue;
+    *a = hipPointerAttribute_t{hipMemoryTypeHost, 0, const_cast<char*>(c), const_cast<char*>(c), 0, 0};
     return hipSuccess;
 }
 inline hipError_t hipMemcpy2DAsync(void* d, size_t dpitch, const void* s, size_t spitch, size_t width,

@@ -5,7 +5,8 @@ shapes that keep every hit list in global scratch instead of LDS (LDS
 capacity 16 hits) with two wavefronts per anchor, the smallest MSA chunk and
 512-anchor device batches, under shapes that merge hit lists of more than 64
 entries list by list instead of sorting them in registers (in LDS and in
-global scratch),
+global scratch), with page-locked result arrays (per-batch copies
+overlapping later batches),
 and under the shapes the autotuner keeps; candidate lists, anchor status
 and corrected anchors must be identical, and the candidate lists equal the
 oracle's.  The CPU tests drive the kernels through the HIP
@@ -98,6 +99,14 @@ def _check(lib, case, d, device_autotune):
             got = eng.tuning()
             assert got["query_lds_hits"] == 16 and got["correct_threads"] == 128 and got["msa_chunk"] == 16
             assert got["batch_anchors"] == 512 and eng.stats().launches == (rs.n + 511) // 512
+            # page-locked result arrays: each batch's results are copied while
+            # the next batches run; they must equal the pageable call's
+            pitch = (rs.max_length + 15) // 16 * 16
+            ps = care_amd.pinned_empty((rs.n,), np.uint8, lib=eng.lib)
+            pq = care_amd.pinned_empty((rs.n, pitch), np.uint8, lib=eng.lib)
+            status, seqs, _, _ = eng.correct(0, rs.n, out=(ps, pq))
+            assert np.array_equal(status, res["status"])
+            assert np.array_equal(seqs[(status & 1) != 0, :rs.max_length], res["rows"])
         if chosen is not None:
             assert chosen["correct_threads"] in (128, 192, 256)
             assert chosen["query_lds_hits"] % 16 == 0 and chosen["correct_blocks_per_cu"] >= 1
