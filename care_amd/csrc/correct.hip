@@ -994,21 +994,27 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 float* ft = reinterpret_cast<float*>(tileScr + 2 * kMlpTileScratch);  // [32][33] features
                 for (int g0 = 0; g0 < nd; g0 += 32) {
                     const int ng = min(32, nd - g0);
-                    if (tid < 32) {
-                        float* f = ft + tid * (kMlpIn + 1);
-                        if (tid < ng) {
-                            const int i = int(L.dec[g0 + tid]);
+                    // thread and wave indices made opaque: the feature-tile
+                    // and MLP addresses are computed here rather than hoisted
+                    // out of the anchor loop (which would keep them in
+                    // registers through every phase and cost a wave per SIMD)
+                    int gt = tid, gw = wave;
+                    asm volatile("" : "+v"(gt), "+v"(gw));
+                    if (gt < 32) {
+                        float* f = ft + gt * (kMlpIn + 1);
+                        if (gt < ng) {
+                            const int i = int(L.dec[g0 + gt]);
                             const float qwi = P.useQuality ? L.qw[(unsigned char)L.anchorQ[i]] : 1.0f;
                             gate_features(L, msa.cols, msa.ab, la, i, nd, m, acs, P.estimatedCoverage, qwi,
                                           [&](int ii) { return base_at(L.aw, ii); }, f);
                         } else {
                             for (int k = 0; k < kMlpIn; k++) f[k] = 0.0f;
                         }
-                        for (int k = 0; k < kMlpIn; k++) xt[tid * kMlpIn + k] = f2bf(f[k]);
-                        if (P.gateFeat && !P.gateRecordCands && tid < ng) {
+                        for (int k = 0; k < kMlpIn; k++) xt[gt * kMlpIn + k] = f2bf(f[k]);
+                        if (P.gateFeat && !P.gateRecordCands && gt < ng) {
                             const int slot = atomicAdd(P.gateCount, 1);
                             if (slot < P.gateCap) {
-                                const int i = int(L.dec[g0 + tid]);
+                                const int i = int(L.dec[g0 + gt]);
                                 for (int k = 0; k < kMlpIn; k++) P.gateFeat[u64(slot) * kMlpIn + k] = f[k];
                                 P.gateIds[slot] = u32(id);
                                 P.gatePos[slot] = u16(i);
@@ -1028,7 +1034,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                         W.b3 = P.gateParams + kMlpB3;
                         W.w4 = P.gateParams + kMlpW4;
                         W.b4 = P.gateParams[kMlpB4];
-                        mlp_tile_forward(W, xt + 16 * wave * kMlpIn, tileScr + wave * kMlpTileScratch, zt + 16 * wave);
+                        mlp_tile_forward(W, xt + 16 * gw * kMlpIn, tileScr + gw * kMlpTileScratch, zt + 16 * gw);
                     }
                     __syncthreads();
                     if (tid < ng) L.gacc[L.dec[g0 + tid]] = (P.gateParams && zt[tid] >= P.gateLogit) ? 1 : 0;

@@ -80,7 +80,11 @@ __device__ __forceinline__ ushort8 load8(const bf16raw* p) { return *reinterpret
 // Hidden activations are rounded to bf16 between layers; layer 4 runs in
 // FP32.  All 64 lanes must call.
 __device__ __forceinline__ void mlp_tile_forward(const MlpWeights& W, const bf16raw* x, char* scratch, float* z) {
-    const int lane = int(__lane_id());
+    // the lane index is made opaque here so that the compiler computes the
+    // lane-dependent addresses below at the call, instead of hoisting them
+    // out of the caller's loops where they would hold registers throughout
+    int lane = int(__lane_id());
+    asm volatile("" : "+v"(lane));
     const int r = lane & 15, g = lane >> 4;
     bf16raw* h1 = reinterpret_cast<bf16raw*>(scratch);
     bf16raw* h2 = h1 + 16 * kMlpH1;
