@@ -1363,25 +1363,33 @@ int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen)
             }
             return best;
         };
-        auto query = [&](int capL) {
-            launch_query(ctx->rs, ctx->tables, first, n, ctx->maxCand, capL, ctx->queryRegMax, o.exclude_ambiguous,
-                         candLimit,
+        auto query = [&](int capL, int regMax) {
+            launch_query(ctx->rs, ctx->tables, first, n, ctx->maxCand, capL, regMax, o.exclude_ambiguous, candLimit,
                          ctx->candIds.p, ctx->candCount.p, reinterpret_cast<u32*>(ctx->metaA.p), nullptr, s);
         };
-        // query: LDS hit capacities up to the longest possible list
+        // query: LDS hit capacities up to the longest possible list and, for
+        // tables whose lists can exceed 1024 hits (unless pinned by
+        // care_set_tuning), register sorts of up to 1024 (lists beyond are
+        // merged) or up to 2048 hits
         const int full = (ctx->maxCand + 15) / 16 * 16;
-        int bestCap = ctx->queryCap;
+        std::vector<int> regs{ctx->queryRegMax};
+        if (ctx->queryRegMax == 0 && ctx->maxCand > 1024) regs = {1024, 0};
+        int bestCap = ctx->queryCap, bestReg = ctx->queryRegMax;
         float bestQ = 1e30f;
-        for (int capL : {128, 256, 512, 1024, 1536, full}) {
-            if (capL > full || 4 * query_wave_lds_bytes(capL, candLimit) > size_t(kMaxLdsBytes)) continue;
-            const float ms = timed([&] { query(capL); });
-            if (ms < bestQ) {
-                bestQ = ms;
-                bestCap = capL;
+        for (int regMax : regs) {
+            for (int capL : {128, 256, 512, 1024, 1536, full}) {
+                if (capL > full || 4 * query_wave_lds_bytes(capL, candLimit) > size_t(kMaxLdsBytes)) continue;
+                const float ms = timed([&] { query(capL, regMax); });
+                if (ms < bestQ) {
+                    bestQ = ms;
+                    bestCap = capL;
+                    bestReg = regMax;
+                }
             }
         }
         ctx->queryCap = bestCap;
-        query(bestCap);
+        ctx->queryRegMax = bestReg;
+        query(bestCap, bestReg);
         HIP_TRY(hipGetLastError());
         // correction: 2, 3 or 4 wavefronts per anchor, automatic and halved MSA chunks
         P.candIds = ctx->candIds.p;

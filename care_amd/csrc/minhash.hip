@@ -733,7 +733,10 @@ void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, 
     const bool wide = maxCand > 16 * kWave;  // lists may exceed the 16-per-lane network
     const int kernelMax = wide ? 32 * kWave : 16 * kWave;
     regMax = regMax > 0 ? std::min(regMax, kernelMax) : kernelMax;
-    if (wide)
+    // the 32-per-lane instantiation holds about 190 VGPRs (two waves per
+    // SIMD), the 16-per-lane one 128 (four): the wider register sort is
+    // used only when asked for lists of more than 1024 hits
+    if (regMax > 16 * kWave)
         hipLaunchKernelGGL(k_query<32>, dim3(blocks), dim3(threads), lds, stream, rs, t, first, n, maxCand, capL, regMax,
                            excludeAmbiguous, candLimit, candIds, candCount, spill, rawCount);
     else
