@@ -1060,7 +1060,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 } else {
                     b = (L.support[col] > 0.90f && L.origCov[col] <= 2) ? u32(L.cons[col]) : orig;
                 }
-                outSeq[i] = "ACGT"[b];
+                outSeq[i] = base_char(b);
             }
             status = u8(1 | (hq ? 2 : 0));
 
@@ -1123,7 +1123,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                             if (b != orig && !cacc[i]) b = orig;
                         }
                         const int out = rc ? cm.len - 1 - i : i;
-                        dst[out] = "ACGT"[rc ? 3u - b : b];
+                        dst[out] = base_char(rc ? 3u - b : b);
                     }
                 }
             }

@@ -33,12 +33,17 @@ __device__ __forceinline__ char quality_char(const u32* enc, int j, int bits) {
         return char((w >> (8 * (j & 3))) & 0xFFu);  // little-endian byte order
     }
     if (bits == 2) {
+        // the four bin characters '$', '(', '1', '9' packed in one constant:
+        // a string-table lookup would be a memory load per call
         const u32 v = (enc[j >> 4] >> (30 - 2 * (j & 15))) & 3u;
-        return "$(19"[v];
+        return char((0x39312824u >> (8u * v)) & 0xFFu);
     }
     const u32 v = (enc[j >> 5] >> (31 - (j & 31))) & 1u;
     return v ? '5' : '(';
 }
+
+// character of a 2-bit base code ('A', 'C', 'G', 'T'), without a table load
+__device__ __forceinline__ char base_char(u32 b) { return char((0x54474341u >> (8u * b)) & 0xFFu); }
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
