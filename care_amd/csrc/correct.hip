@@ -415,6 +415,9 @@ __device__ ColumnStats column_stats(const Lds& L, int first, int last) {
     float mn = L.support[first];
     float sum = 0.0f;
     int mnc = L.coverage[first], mxc = L.coverage[first];
+    // one thread, in column order (the support sum is sequential FP32 as on
+    // the CPU); unrolled so that the LDS loads run ahead of the add chain
+#pragma unroll 8
     for (int c = first; c < last; c++) {
         const float s = L.support[c];
         mn = s < mn ? s : mn;
