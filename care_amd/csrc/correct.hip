@@ -158,8 +158,15 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
         lmin = min(lmin, s);
         lmax = max(lmax, s + int(meta[c].len));
     }
-    atomicMin(sMin, lmin);
-    atomicMax(sMax, lmax);
+    // wave reductions first: one LDS atomic per wave instead of per thread
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        lmin = min(lmin, __shfl_xor(lmin, o, kWave));
+        lmax = max(lmax, __shfl_xor(lmax, o, kWave));
+    }
+    if (lane_id() == 0) {
+        atomicMin(sMin, lmin);
+        atomicMax(sMax, lmax);
+    }
     __syncthreads();
     Msa msa;
     msa.cols = *sMax - *sMin;
