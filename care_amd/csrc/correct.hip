@@ -883,14 +883,17 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 if (L.cons[msa.ab + i] != u8(base_at(L.aw, i))) sFlag = 1;
             __syncthreads();
             if (!sFlag) break;
+            int firstCol = 0x7FFFFFFF;
             for (int i = tid; i < la; i += blockDim.x) {
                 const int col = msa.ab + i;
                 const int ci = L.cons[col];
                 bool found = false;
                 for (int b = 0; b < 4; b++)
                     if (b != ci && P.sigCountThreshold <= int(L.cnt[b * msa.cols + col])) found = true;
-                if (found) atomicMin(&sCol, col);
+                if (found) firstCol = min(firstCol, col);
             }
+            for (int o = kWave / 2; o > 0; o >>= 1) firstCol = min(firstCol, __shfl_xor(firstCol, o, kWave));
+            if (lane == 0 && firstCol != 0x7FFFFFFF) atomicMin(&sCol, firstCol);
             __syncthreads();
             const int col = sCol;
             if (col == 0x7FFFFFFF) break;
