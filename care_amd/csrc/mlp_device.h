@@ -90,9 +90,11 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpWeights& W, const bf16
     bf16raw* h2 = h1 + 16 * kMlpH1;
     float* h3 = reinterpret_cast<float*>(h2 + 16 * kMlpH2);
     const ushort8 a1 = load8(x + r * kMlpIn + 8 * g);
-    // loops kept rolled: the gate runs inside the register-heavy
-    // correction kernel, where unrolled weight loads would lower occupancy
-#pragma unroll 1
+    // loops unrolled so that the weight loads of a layer are in flight
+    // together (the weights come from L2; one load per MFMA in sequence
+    // would serialise their latencies); with the lane index opaque the
+    // unrolled loads do not raise the correction kernel's register peak
+#pragma unroll
     for (int t = 0; t < kMlpH1 / 16; t++) {
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
         acc = mfma32(a1, load8(W.w1 + (16 * t + r) * kMlpIn + 8 * g), acc);
@@ -105,10 +107,10 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpWeights& W, const bf16
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll 1
+#pragma unroll
     for (int t = 0; t < kMlpH2 / 16; t++) {
         floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
+#pragma unroll
         for (int kk = 0; kk < kMlpH1 / 32; kk++)
             acc = mfma32(load8(h1 + r * kMlpH1 + 32 * kk + 8 * g), load8(W.w2 + (16 * t + r) * kMlpH1 + 32 * kk + 8 * g),
                          acc);
@@ -135,7 +137,7 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpWeights& W, const bf16
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (lane < 16) {
         float s = W.b4;
-#pragma unroll 4
+#pragma unroll
         for (int j = 0; j < kMlpH3; j++) s = fmaf(h3[lane * kMlpH3 + j], W.w4[j], s);
         z[lane] = s;
     }
