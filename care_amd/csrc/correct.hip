@@ -66,6 +66,7 @@ struct Lds {
     u32* aw;        // anchor 2-bit words [pitch]
     uint2* ap;      // anchor planes [hw + 3]
     char* anchorQ;  // [maxLen]
+    u32* keys;      // [chunk][2] alignment keys (forward, reverse complement) of the staged candidates
     uint2* cpl;     // [chunk] candidate planes: forward, then reverse complement, 2 x [hw + 3]
     u8* cons;       // [cols]
     float* support;
@@ -94,6 +95,7 @@ struct LdsDims {
         b += sizeof(float) * cols + 2 * sizeof(int) * cols;
         b += 4 * sizeof(u16) * cols + 4 * sizeof(float) * cols;
         b += 32 * sizeof(int);
+        b += 2 * sizeof(u32) * size_t(chunk);
         if (gated & 1) b += (sizeof(u16) + 1) * ((size_t(maxLen) + 15) / 16 * 16);
         return b;
     }
@@ -118,6 +120,7 @@ __device__ Lds carve(char* base, const LdsDims& d) {
     l.cnt = reinterpret_cast<u16*>(p); p += 4 * sizeof(u16) * d.cols;
     l.cons = reinterpret_cast<u8*>(p); p += (size_t(d.cols) + 3) / 4 * 4;
     l.anchorQ = reinterpret_cast<char*>(p); p += (size_t(d.maxLen) + 3) / 4 * 4;
+    l.keys = reinterpret_cast<u32*>(p); p += 2 * sizeof(u32) * size_t(d.chunk);
     l.gscr = reinterpret_cast<char*>(l.mst);
     if (d.gated & 1) {
         const size_t ml = (size_t(d.maxLen) + 15) / 16 * 16;
@@ -382,6 +385,27 @@ __device__ __forceinline__ int choose_orientation(const Aln& f, const Aln& r, in
 
 // include/forest.hpp:57-68, 211-226: mean leaf probability over the trees,
 // accepting as soon as the running mean reaches the threshold
+// candidate record from its staged id / length / pair flag and the two
+// alignment keys: orientation chosen, shift, overlap and mismatches of it
+__device__ __forceinline__ CandMeta aligned_meta(const CandMeta& st, u32 kf, u32 kr, int la, int minov,
+                                                 const CorrectParams& P) {
+    const int lc = st.len;
+    const Aln f = decode_key(kf, la, lc, minov);
+    const Aln r = decode_key(kr, la, lc, minov);
+    const int flag = choose_orientation(f, r, la, P.minOverlapRatio, P.minOverlap, P.maxErr4);
+    const Aln& ch = flag == 2 ? r : f;
+    CandMeta m;
+    m.id = st.id;
+    m.shift = short(ch.shift);
+    m.len = u16(lc);
+    m.u.al.nOps = u16(ch.nOps);
+    m.u.al.overlap = u16(ch.overlap);
+    m.flag = u8(flag);
+    m.paired = st.paired;
+    m.pad = 0;
+    return m;
+}
+
 __device__ bool forest_decide(const DevForest& F, const float* f) {
     float sum = 0.f;
     for (int t = 0; t < F.numTrees; t++) {
@@ -776,30 +800,31 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 L.cpl[size_t(2 * c + 1) * pw + p] = make_uint2(hi, lo);
             }
             __syncthreads();
+            // the plain kernel decodes the keys and chooses orientations for
+            // the whole chunk at once, one thread per candidate (its chunks
+            // hold about 50 candidates); the gated one (at most candidate_limit
+            // candidates, a few per wave) decodes in each wave's lane 0 and
+            // saves the extra barrier
             for (int c = wave; c < nc; c += nwaves) {
-                const u32 cid = L.mst[c].id;
                 const int lc = L.mst[c].len;
                 const uint2* fp = L.cpl + size_t(2 * c) * pw;
                 const uint2* rp = fp + pw;
                 u32 kf, kr;
                 align_wave(L.ap, la, fp, rp, lc, minovA, P.maxErrorRate, &kf, &kr);
                 if (lane == 0) {
-                    const Aln f = decode_key(kf, la, lc, minovA);
-                    const Aln r = decode_key(kr, la, lc, minovA);
-                    const int flag = choose_orientation(f, r, la, P.minOverlapRatio, P.minOverlap, P.maxErr4);
-                    const Aln& ch = flag == 2 ? r : f;
-                    CandMeta m;
-                    m.id = cid;
-                    m.shift = short(ch.shift);
-                    m.len = u16(lc);
-                    m.u.al.nOps = u16(ch.nOps);
-                    m.u.al.overlap = u16(ch.overlap);
-                    m.flag = u8(flag);
-                    m.paired = L.mst[c].paired;
-                    m.pad = 0;
-                    metaA[c0 + c] = m;
+                    if (kGated) {
+                        metaA[c0 + c] = aligned_meta(L.mst[c], kf, kr, la, minovA, P);
+                    } else {
+                        L.keys[2 * c] = kf;
+                        L.keys[2 * c + 1] = kr;
+                    }
                 }
                 wave_sync();
+            }
+            if (!kGated) {
+                __syncthreads();
+                for (int c = tid; c < nc; c += blockDim.x)
+                    metaA[c0 + c] = aligned_meta(L.mst[c], L.keys[2 * c], L.keys[2 * c + 1], la, minovA, P);
             }
         }
         __syncthreads();
