@@ -139,8 +139,8 @@ int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threa
  * time (src/gpu/alignmentkernels.cu:652, src/gpu/msakernels.cu:1186). */
 typedef struct care_tuning {
     int batch_anchors;          /* anchors per internal launch, even (0 = automatic) */
-    int correct_threads;        /* threads per correction workgroup, i.e. 2 or 4 wavefronts
-                                   per anchor: 128, 192 or 256 (0 = 256) */
+    int correct_threads;        /* threads per correction workgroup, i.e. 2, 3 or 4
+                                   wavefronts per anchor: 128, 192 or 256 (0 = 256) */
     int correct_blocks_per_cu;  /* persistent correction workgroups per compute unit
                                    (0 = as many as are resident at once) */
     int msa_chunk;              /* candidates staged in LDS per MSA pass, 16..256
@@ -151,7 +151,9 @@ typedef struct care_tuning {
     int query_register_hits;    /* longest hit list sorted in registers (bitonic network
                                    across the wavefront), power of two in 64..2048;
                                    longer lists are merged list by list (0 = 1024, or
-                                   2048 when num_hash_maps * bucket cap exceeds 1024) */
+                                   2048 when num_hash_maps * bucket cap exceeds 1024);
+                                   up to 1024 runs the 16-per-lane query kernel (fewer
+                                   registers), above it the 32-per-lane one */
 } care_tuning;
 
 int care_set_tuning(care_context* ctx, const care_tuning* tuning);
