@@ -1323,7 +1323,7 @@ extern "C" {
 
 // Shape search: the query kernel over the LDS hit capacities that fit,
 // then the correction kernel over workgroup sizes and MSA staging chunks,
-// each timed twice (minimum kept) on the same sample of anchors.
+// each timed three times (minimum kept) on the same sample of anchors.
 int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen) {
     if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
     if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
@@ -1351,7 +1351,7 @@ int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen)
         const int candLimit = o.correction_type == 3 ? o.candidate_limit : 0;
         auto timed = [&](auto&& launch) {
             float best = 1e30f;
-            for (int rep = 0; rep < 2; rep++) {
+            for (int rep = 0; rep < 3; rep++) {
                 HIP_TRY(hipEventRecord(ctx->ev[0], s));
                 launch();
                 HIP_TRY(hipGetLastError());

@@ -53,12 +53,20 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Minimum over the 64 lanes, returned in every lane; all lanes must be
+// active.  Within each row of 16 lanes by DPP moves (quad_perm [1,0,3,2] and
+// [2,3,0,1], i.e. lane ^ 1 and lane ^ 2; row_half_mirror, lane i of an
+// 8-lane half-row taking lane 7 - i; row_mirror, lane i of a row taking lane
+// 15 - i), then across the four rows by readlane: no LDS round trips, unlike
+// shuffles, which compile to ds_bpermute.
 __device__ __forceinline__ u32 wave_min_u32(u32 v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const u32 w = __shfl_xor(v, o, kWave);
-        v = w < v ? w : v;
-    }
-    return v;
+    v = min(v, u32(__builtin_amdgcn_mov_dpp(int(v), 0xB1, 0xF, 0xF, true)));
+    v = min(v, u32(__builtin_amdgcn_mov_dpp(int(v), 0x4E, 0xF, 0xF, true)));
+    v = min(v, u32(__builtin_amdgcn_mov_dpp(int(v), 0x141, 0xF, 0xF, true)));
+    v = min(v, u32(__builtin_amdgcn_mov_dpp(int(v), 0x140, 0xF, 0xF, true)));
+    const u32 r0 = u32(__builtin_amdgcn_readlane(int(v), 0)), r1 = u32(__builtin_amdgcn_readlane(int(v), 16));
+    const u32 r2 = u32(__builtin_amdgcn_readlane(int(v), 32)), r3 = u32(__builtin_amdgcn_readlane(int(v), 48));
+    return min(min(r0, r1), min(r2, r3));
 }
 
 __device__ __forceinline__ int wave_sum_int(int v) {

@@ -399,6 +399,21 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     (void)width;
     return emu::exchange(v, (emu::lane() ^ mask) & 63);
 }
+// DPP moves used by the kernels (source lane within the 16-lane row):
+// quad_perm 0x00-0xFF, row_mirror 0x140, row_half_mirror 0x141
+inline int __builtin_amdgcn_mov_dpp(int v, int ctrl, int, int, bool) {
+    const int l = emu::lane(), row = l & ~15, i = l & 15;
+    int src;
+    if (ctrl >= 0 && ctrl <= 0xFF) src = (l & ~3) | ((ctrl >> (2 * (l & 3))) & 3);
+    else if (ctrl == 0x140) src = row | (15 - i);
+    else if (ctrl == 0x141) src = row | (i & 8) | (7 - (i & 7));
+    else {
+        std::fprintf(stderr, "emulated mov_dpp: control 0x%x not emulated\n", ctrl);
+        std::abort();
+    }
+    return emu::exchange(v, src);
+}
+inline int __builtin_amdgcn_readlane(int v, int lane) { return emu::exchange(v, lane & 63); }
 template <class T>
 inline T __shfl_up(T v, unsigned delta, int width = 64) {
     (void)width;
