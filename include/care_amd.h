@@ -120,6 +120,9 @@ typedef struct care_batch_result {
 } care_batch_result;
 
 /* Corrects anchors [first, first+n) (n even for paired-end, first even).
+ * Result arrays may be any host memory; into page-locked ones
+ * (care_host_buffer_alloc) each internal batch's anchor results are copied
+ * while later batches still run.  Returns when all results are in place.
  * Replaces GpuErrorCorrector::correct (include/gpu/gpucorrector.cuh:1060)
  * with its kernels: alignment, filters, MSA build + refinement, anchor and
  * candidate correction (src/gpu/{alignment,msa,anchorcorrection,
