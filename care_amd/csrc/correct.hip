@@ -162,10 +162,8 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
         lmax = max(lmax, s + int(meta[c].len));
     }
     // wave reductions first: one LDS atomic per wave instead of per thread
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        lmin = min(lmin, __shfl_xor(lmin, o, kWave));
-        lmax = max(lmax, __shfl_xor(lmax, o, kWave));
-    }
+    lmin = wave_min_int(lmin);
+    lmax = wave_max_int(lmax);
     if (lane_id() == 0) {
         atomicMin(sMin, lmin);
         atomicMax(sMax, lmax);
@@ -180,10 +178,14 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
     const int cols = msa.cols;
     const int pitch = rs.pitch, qpitch = useQ ? rs.qpitch : 0;
     // staging assignment: thread -> (row r0 + k * rowsPerStep, word w)
+    // thread index opaque: the staging assignment is computed per build, not
+    // hoisted out of the anchor loop into registers held by every phase
+    int tx = int(threadIdx.x);
+    asm volatile("" : "+v"(tx));
     const int rowsS = int(blockDim.x) / pitch;
-    const int r0S = int(threadIdx.x) / pitch, wS = int(threadIdx.x) - r0S * pitch;
+    const int r0S = tx / pitch, wS = tx - r0S * pitch;
     const int rowsQ = qpitch ? int(blockDim.x) / qpitch : 0;
-    const int r0Q = qpitch ? int(threadIdx.x) / qpitch : 0, wQ = int(threadIdx.x) - r0Q * qpitch;
+    const int r0Q = qpitch ? tx / qpitch : 0, wQ = tx - r0Q * qpitch;
     const int wb = max(0, msa.ab - margin), we = min(cols, ae + margin);
     for (int cb = wb; cb < we; cb += blockDim.x) {
         const int col = cb + int(threadIdx.x);
@@ -917,7 +919,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     if (b != ci && P.sigCountThreshold <= int(L.cnt[b * msa.cols + col])) found = true;
                 if (found) firstCol = min(firstCol, col);
             }
-            for (int o = kWave / 2; o > 0; o >>= 1) firstCol = min(firstCol, __shfl_xor(firstCol, o, kWave));
+            firstCol = wave_min_int(firstCol);
             if (lane == 0 && firstCol != 0x7FFFFFFF) atomicMin(&sCol, firstCol);
             __syncthreads();
             const int col = sCol;

@@ -69,10 +69,22 @@ __device__ __forceinline__ u32 wave_min_u32(u32 v) {
     return min(min(r0, r1), min(r2, r3));
 }
 
-__device__ __forceinline__ int wave_sum_int(int v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+// Reduction over the 64 lanes with the associative, commutative `op`,
+// returned in every lane; all lanes must be active.  Same DPP / readlane
+// pattern as wave_min_u32.
+template <class T, class Op>
+__device__ __forceinline__ T wave_reduce(T v, Op op) {
+    v = op(v, T(__builtin_amdgcn_mov_dpp(int(v), 0xB1, 0xF, 0xF, true)));
+    v = op(v, T(__builtin_amdgcn_mov_dpp(int(v), 0x4E, 0xF, 0xF, true)));
+    v = op(v, T(__builtin_amdgcn_mov_dpp(int(v), 0x141, 0xF, 0xF, true)));
+    v = op(v, T(__builtin_amdgcn_mov_dpp(int(v), 0x140, 0xF, 0xF, true)));
+    const T r0 = T(__builtin_amdgcn_readlane(int(v), 0)), r1 = T(__builtin_amdgcn_readlane(int(v), 16));
+    const T r2 = T(__builtin_amdgcn_readlane(int(v), 32)), r3 = T(__builtin_amdgcn_readlane(int(v), 48));
+    return op(op(r0, r1), op(r2, r3));
 }
+__device__ __forceinline__ int wave_sum_int(int v) { return wave_reduce(v, [](int x, int y) { return x + y; }); }
+__device__ __forceinline__ int wave_min_int(int v) { return wave_reduce(v, [](int x, int y) { return x < y ? x : y; }); }
+__device__ __forceinline__ int wave_max_int(int v) { return wave_reduce(v, [](int x, int y) { return x > y ? x : y; }); }
 
 // Exclusive prefix count of `flag` over the whole block, in thread order.
 // `scratch` must hold blockDim.x/64 + 1 ints.  Returns the prefix for this
