@@ -413,6 +413,25 @@ int care_readstore_upload(care_context* ctx, uint64_t num_reads, int max_length,
     }
     for (uint64_t i = 0; i < num_reads; i++)
         if (lengths[i] < 1 || lengths[i] > max_length) return fail(CARE_ERR_ARGUMENT, "read %llu has length %d", (unsigned long long)i, lengths[i]);
+    // The device store always uses the minimal pitches (the kernels size
+    // their LDS rows from max_length); wider caller rows are repacked.
+    const int minPitch = (max_length + 15) / 16;
+    const int minQPitch = ctx->opt.use_quality ? (qual_bits * max_length + 31) / 32 : 0;
+    std::vector<u32> packedSeqs, packedQuals;
+    if (seq_pitch != minPitch) {
+        packedSeqs.resize(size_t(num_reads) * minPitch);
+        for (uint64_t i = 0; i < num_reads; i++)
+            std::memcpy(packedSeqs.data() + i * minPitch, seqs + i * size_t(seq_pitch), sizeof(u32) * minPitch);
+        seqs = packedSeqs.data();
+        seq_pitch = minPitch;
+    }
+    if (ctx->opt.use_quality && qual_pitch != minQPitch) {
+        packedQuals.resize(size_t(num_reads) * minQPitch);
+        for (uint64_t i = 0; i < num_reads; i++)
+            std::memcpy(packedQuals.data() + i * minQPitch, quals + i * size_t(qual_pitch), sizeof(u32) * minQPitch);
+        quals = packedQuals.data();
+        qual_pitch = minQPitch;
+    }
     try {
         HIP_TRY(hipSetDevice(ctx->device));
         ctx->seqs.alloc(num_reads * seq_pitch);

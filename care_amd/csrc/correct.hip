@@ -130,6 +130,10 @@ __device__ Lds carve(char* base, const LdsDims& d) {
         l.dec = nullptr;
         l.gacc = nullptr;
     }
+    CARE_CHECK(size_t(p - base) + ((d.gated & 1) ? (size_t(d.maxLen) + 15) / 16 * 16 : 0) <= d.bytes());
+#ifdef CARE_EMU_CHECKS
+    CARE_CHECK(d.bytes() <= emu::dyn_lds_bytes());
+#endif
     return l;
 }
 
@@ -172,11 +176,13 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
     Msa msa;
     msa.cols = *sMax - *sMin;
     msa.ab = -*sMin;
+    CARE_CHECK(msa.cols <= dims.cols && msa.ab >= 0);
     const int ae = msa.ab + la;
     const DevReadStore& rs = P.rs;
     const bool useQ = P.useQuality != 0;
     const int cols = msa.cols;
     const int pitch = rs.pitch, qpitch = useQ ? rs.qpitch : 0;
+    CARE_CHECK(pitch == dims.pitch && qpitch == dims.qpitch);
     // staging assignment: thread -> (row r0 + k * rowsPerStep, word w)
     // thread index opaque: the staging assignment is computed per build, not
     // hoisted out of the anchor loop into registers held by every phase
@@ -218,7 +224,10 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
         for (int c0 = 0; c0 < m; c0 += dims.chunk) {
             const int nc = min(dims.chunk, m - c0);
             __syncthreads();
-            for (int i = threadIdx.x; i < nc; i += blockDim.x) store_meta(L.mst + i, load_meta(meta + c0 + i));
+            for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+                CARE_CHECK(c0 + i < P.maxCand && meta[c0 + i].id < rs.n);
+                store_meta(L.mst + i, load_meta(meta + c0 + i));
+            }
             if (r0S < rowsS)
                 for (int c = r0S; c < nc; c += rowsS)
                     L.rows[c * pitch + wS] = rs.seqs[u64(meta[c0 + c].id) * pitch + wS];
@@ -666,8 +675,10 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         if (a >= P.numAnchors) break;
 
         const u64 id = P.firstAnchor + u64(a);
+        CARE_CHECK(id < rs.n);
         const int la = rs.lens[id];
         const int n = P.candCount[a];
+        CARE_CHECK(la >= 1 && la <= dims.maxLen && n >= 0 && n <= P.maxCand && rs.pitch == dims.pitch);
         const u32* ids = P.candIds + u64(a) * P.maxCand;
         u8 status = 0;
         char* outSeq = P.outSeq + u64(a) * P.outPitch;
@@ -728,6 +739,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             for (int i = tid; i < nc * rs.pitch; i += blockDim.x) {
                 const int c = i / rs.pitch, w = i - c * rs.pitch;
                 const u32 cid = ids[c0 + c];
+                CARE_CHECK(cid < rs.n && cid != u32(id) && c < dims.chunk);
                 L.rows[i] = rs.seqs[u64(cid) * rs.pitch + w];
                 if (w == 0) {
                     L.mst[c].id = cid;
@@ -737,7 +749,8 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     u8 paired = 0;
                     if (mateIds) {
                         const u32 want = cid ^ 1u;
-                        int lo2 = 0, hi2 = mateN;
+                        CARE_CHECK(mateN >= 0 && mateN <= P.maxCand);
+                    int lo2 = 0, hi2 = mateN;
                         while (lo2 < hi2) {
                             const int mid = (lo2 + hi2) >> 1;
                             if (mateIds[mid] < want) lo2 = mid + 1;
@@ -814,6 +827,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 u32 kf, kr;
                 align_wave(L.ap, la, fp, rp, lc, minovA, P.maxErrorRate, &kf, &kr);
                 if (lane == 0) {
+                    CARE_CHECK(c0 + c < n);
                     if (kGated) {
                         metaA[c0 + c] = aligned_meta(L.mst[c], kf, kr, la, minovA, P);
                     } else {
@@ -879,6 +893,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             int cnt;
             const int pos = block_exclusive_count(keep, L.scratch, &cnt);
             if (keep) {
+                CARE_CHECK(m + pos < n && cm.u.al.overlap > 0);
                 const float w = 1.0f - sqrtf(float(cm.u.al.nOps) / (float(cm.u.al.overlap) * P.maxErrorRate));
                 cm.u.weight = w;
                 store_meta(metaB + m + pos, cm);
@@ -966,6 +981,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 }
                 int cnt;
                 const int pos = block_exclusive_count(keep, L.scratch, &cnt);
+                CARE_CHECK(!keep || m2 + pos < m);
                 if (keep) store_meta(other + m2 + pos, cm);
                 m2 += cnt;
             }
@@ -1024,6 +1040,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     if (i < la) L.gacc[i] = 0;
                     int cnt;
                     const int pos = block_exclusive_count(d, L.scratch, &cnt);
+                    CARE_CHECK(!d || nd + pos < la);
                     if (d) L.dec[nd + pos] = u16(i);
                     nd += cnt;
                 }
@@ -1032,6 +1049,8 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 float* zt = reinterpret_cast<float*>(L.gscr + 32 * kMlpIn * 2);   // [32]
                 char* tileScr = L.gscr + 32 * kMlpIn * 2 + 32 * 4;                // 2 x kMlpTileScratch
                 float* ft = reinterpret_cast<float*>(tileScr + 2 * kMlpTileScratch);  // [32][33] features
+                // the tiles overlay the candidate staging area, which ends at qw
+                CARE_CHECK(reinterpret_cast<char*>(ft + 32 * (kMlpIn + 1)) <= reinterpret_cast<char*>(L.qw));
                 for (int g0 = 0; g0 < nd; g0 += 32) {
                     const int ng = min(32, nd - g0);
                     // thread and wave indices made opaque: the feature-tile
@@ -1125,6 +1144,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     if (lane == 0) slot = atomicAdd(P.candOutCount, 1);
                     slot = __shfl(slot, 0, kWave);
                     if (slot >= P.candOutCap) continue;
+                    CARE_CHECK(cm.id < rs.n && qb >= 0 && qe <= msa.cols);
                     if (lane == 0) P.candOutIds[slot] = cm.id;
                     ColumnStats ccs{};
                     if (forestC || gateC) {
@@ -1142,6 +1162,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     u8* cacc = nullptr;
                     if (gateC) {
                         char* ws = L.gscr + size_t(wave) * cand_gate_scratch(dims.maxLen);
+                        CARE_CHECK(ws + cand_gate_scratch(dims.maxLen) <= reinterpret_cast<char*>(L.qw));
                         cacc = reinterpret_cast<u8*>(ws + kCandGateTile);
                         gate_candidate(P, L, rs, msa.cols, qb, cm, m, ccs, ws, cacc);
                     }

@@ -3,6 +3,15 @@
 
 #include "engine.h"
 
+// Index checks of the kernels: compiled only into the CPU emulation of the
+// HIP API built with CARE_EMU_CHECKS (test infrastructure, where a failed
+// check aborts with its location); nothing in the device build.
+#ifdef CARE_EMU_CHECKS
+#define CARE_CHECK(cond) emu::check((cond), #cond, __FILE__, __LINE__)
+#else
+#define CARE_CHECK(cond) ((void)0)
+#endif
+
 namespace care_engine {
 
 constexpr int kWave = 64;

@@ -537,6 +537,7 @@ __device__ __forceinline__ int compact_unique(Buf sorted, int total, u32 self, c
         const u64 bal = __ballot(keep);
         const int pos = written + __popcll(bal & below);
         if (keep) {
+            CARE_CHECK(v < rs.n && pos < total);
             out[pos] = v;
             if (candLimit > 0) {
                 // maps that hit v: length of its run of equal entries (at most
@@ -619,6 +620,9 @@ __global__ void __launch_bounds__(256) k_query(DevReadStore rs, DevTables t, u64
     u32* la = reinterpret_cast<u32*>(wl + kQueryHeader);            // capL
     u32* lb = la + capL;                                             // capL
     u8* lh = reinterpret_cast<u8*>(lb + capL);                       // capL (candLimit > 0)
+#ifdef CARE_EMU_CHECKS
+    CARE_CHECK(size_t(wpb) * query_wave_lds_bytes(capL, candLimit) <= emu::dyn_lds_bytes() && h + 1 <= 72);
+#endif
 
     for (int a = blockIdx.x * wpb + wave; a < n; a += gridDim.x * wpb) {
         const u64 id = first + u64(a);
@@ -655,6 +659,7 @@ __global__ void __launch_bounds__(256) k_query(DevReadStore rs, DevTables t, u64
             if (lane >= o) incl += v;
         }
         const int total = __shfl(incl, kWave - 1, kWave);
+        CARE_CHECK(total <= maxCand && id < rs.n);
         if (lane < h) {
             counts[lane] = incl - c;
             lists[lane] = list;

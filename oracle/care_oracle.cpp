@@ -6,11 +6,13 @@
 //
 // It restates, in plain C++17 written for this project, the algorithm of the
 // reference CPU corrector.  Every function names the reference file:line it
-// follows.  It is pinned three ways: (1) end-to-end FASTQ output bit-identical
-// to care-cpu built from the reference sources (oracle/ref.mk) on seeded
-// inputs, (2) committed golden fixtures under tests/golden/, and (3) closed
-// forms / brute force in the tests (character-level shifted Hamming, brute
-// force minhash, encode/decode round trips).
+// follows.  It is pinned by closed forms, known answers and brute force in
+// tests/test_oracle.py and tests/test_oracle_pins.py (codecs, quality bins,
+// murmur finalizer against the published MurmurHash3 verification value,
+// brute-force minhash and tables, character-level shifted Hamming,
+// hand-built pileups for the MSA, refinement, HQ, filter, edit, forest and
+// combine rules).  The reference care-cpu is not buildable here: its
+// include/helpers points into an un-vendored git submodule.
 //
 // Build (see oracle/build.sh): g++ -O2 -std=c++17 -fopenmp -ffp-contract=off
 
@@ -887,6 +889,40 @@ struct MSA {
 };
 
 // ---------------------------------------------------------------------------
+// Single-end mismatch-ratio filter (include/corrector.hpp:1053-1149): the
+// threshold is 2, 3 or 4 x the estimated error rate, the smallest whose
+// count of candidates below it reaches coverage x m_coverage; with no
+// candidate below 4x, or none reaching it at any level, nothing passes.
+// ---------------------------------------------------------------------------
+float seFilterThreshold(const float* ratios, int n, float estimatedErrorrate, float estimatedCoverage, float mCoverage) {
+    const float base = estimatedErrorrate * 1.0f;
+    const float goodT = estimatedCoverage * mCoverage;
+    int counts[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        if (ratios[i] < 2 * base) counts[0]++;
+        if (ratios[i] < 3 * base) counts[1]++;
+        if (ratios[i] < 4 * base) counts[2]++;
+    }
+    float thr = std::numeric_limits<float>::lowest();
+    if (counts[0] > 0 || counts[1] > 0 || counts[2] > 0) {
+        if (counts[0] >= goodT) thr = 2 * base;
+        else if (counts[1] >= goodT) thr = 3 * base;
+        else if (counts[2] >= goodT) thr = 4 * base;
+    }
+    return thr;
+}
+
+// Candidates that candidate correction writes (MSA columns [qb, qe), anchor
+// columns [ab, ae), ncc = new columns to correct): classic
+// (src/msa.cpp:457-522) needs the start within +-ncc of the anchor start and
+// the end at most ncc past the anchor end; the forest and the learned gate
+// (include/corrector.hpp:1539-1623) only bound the start from below.
+bool candidateInRange(int qb, int qe, int ab, int ae, int ncc, bool forestOrGate) {
+    if (forestOrGate) return qb >= ab - ncc && qe <= ae + ncc;
+    return ab - ncc <= qb && qb <= ab + ncc && qe <= ae + ncc;
+}
+
+// ---------------------------------------------------------------------------
 // Per-read results and the combine rule (src/correctionresultoutput.cpp:730-1023)
 // ---------------------------------------------------------------------------
 struct Result {
@@ -1111,21 +1147,10 @@ struct Corrector {
                     if (c.paired || ratio < opt.pairedFilterThreshold) kept.push_back(std::move(c));
                 }
             } else {
-                const float base = opt.estimatedErrorrate * 1.0f;
-                const float goodT = opt.estimatedCoverage * opt.m_coverage;
-                int counts[3] = {0, 0, 0};
-                for (auto& c : cands) {
-                    const float ratio = float(c.al.nOps) / float(c.al.overlap);
-                    if (ratio < 2 * base) counts[0]++;
-                    if (ratio < 3 * base) counts[1]++;
-                    if (ratio < 4 * base) counts[2]++;
-                }
-                float thr = std::numeric_limits<float>::lowest();
-                if (counts[0] > 0 || counts[1] > 0 || counts[2] > 0) {
-                    if (counts[0] >= goodT) thr = 2 * base;
-                    else if (counts[1] >= goodT) thr = 3 * base;
-                    else if (counts[2] >= goodT) thr = 4 * base;
-                }
+                std::vector<float> ratios;
+                for (auto& c : cands) ratios.push_back(float(c.al.nOps) / float(c.al.overlap));
+                const float thr = seFilterThreshold(ratios.data(), int(ratios.size()), opt.estimatedErrorrate,
+                                                    opt.estimatedCoverage, opt.m_coverage);
                 for (auto& c : cands) {
                     const float ratio = float(c.al.nOps) / float(c.al.overlap);
                     if (ratio < thr) kept.push_back(std::move(c));
@@ -1265,9 +1290,7 @@ struct Corrector {
                 // the engine's learned gate on candidate columns (gated mode):
                 // the forest's range and rule, the gate deciding
                 const bool gateC = opt.correctionType == 3 && opt.correctionTypeCands == 3;
-                const bool inRange = (forest || gateC) ? (qb >= msa.anchorBegin - ncc && qe <= msa.anchorEnd + ncc)
-                                                       : (msa.anchorBegin - ncc <= qb && qb <= msa.anchorBegin + ncc &&
-                                                          qe <= msa.anchorEnd + ncc);
+                const bool inRange = candidateInRange(qb, qe, msa.anchorBegin, msa.anchorEnd, ncc, forest || gateC);
                 if (!inRange) continue;
                 std::string seq(msa.consensus.begin() + qb, msa.consensus.begin() + qe);
                 if (forest) {
@@ -1796,6 +1819,145 @@ int orc_tables_query_limited(void* t, const std::uint32_t* enc, int len, std::ui
     for (int i = 0; i < n && i < cap; i++) out[i] = hc[std::size_t(i)].second;
     return n;
 }
+// ---- single rules, for the known-answer tests ----
+
+namespace {
+orc::MSA msaFrom(int useQ, int la, const char* anchor, const char* anchorQ, int n, const char* cands, int candPitch,
+                 const char* candQ, int qualPitch, const int* lens, const int* shifts, const float* weights) {
+    orc::MSA msa;
+    orc::MSA::Input in{useQ != 0, la, n, candPitch, qualPitch, anchor, cands, anchorQ, candQ, lens, shifts, weights};
+    msa.build(in);
+    return msa;
+}
+}  // namespace
+
+// MSA of an anchor and candidates given as characters (src/msa.cpp:9-173):
+// dims = {nColumns, anchorBegin}; column arrays hold cap columns (counts and
+// weights [4][cap] in A, C, G, T order).  Returns -1 if cap is too small.
+int orc_msa_build(int useQ, int la, const char* anchor, const char* anchorQ, int n, const char* cands, int candPitch,
+                  const char* candQ, int qualPitch, const int* lens, const int* shifts, const float* weights, int cap,
+                  int* dims, char* consensus, float* support, int* coverage, int* counts, float* wts, int* origCov) {
+    const orc::MSA m = msaFrom(useQ, la, anchor, anchorQ, n, cands, candPitch, candQ, qualPitch, lens, shifts, weights);
+    dims[0] = m.nColumns;
+    dims[1] = m.anchorBegin;
+    if (m.nColumns > cap) return -1;
+    for (int c = 0; c < m.nColumns; c++) {
+        consensus[c] = m.consensus[c];
+        support[c] = m.support[c];
+        coverage[c] = m.coverage[c];
+        counts[0 * cap + c] = m.cA[c];
+        counts[1 * cap + c] = m.cC[c];
+        counts[2 * cap + c] = m.cG[c];
+        counts[3 * cap + c] = m.cT[c];
+        wts[0 * cap + c] = m.wA[c];
+        wts[1 * cap + c] = m.wC[c];
+        wts[2 * cap + c] = m.wG[c];
+        wts[3 * cap + c] = m.wT[c];
+        origCov[c] = m.origCoverages[c];
+    }
+    return 0;
+}
+
+// MSA properties over columns [first, last) of the given support and
+// coverage (src/msa.cpp:304-376): out4 = {avg support, min support, min
+// coverage, max coverage}; returns the high-quality flag.
+int orc_msa_properties(const float* support, const int* coverage, int ncols, int first, int last, float err, float cov,
+                       float mcov, float* out4) {
+    orc::MSA m;
+    m.nColumns = ncols;
+    m.support.assign(support, support + ncols);
+    m.coverage.assign(coverage, coverage + ncols);
+    const auto p = m.properties(first, last, err, cov, mcov);
+    out4[0] = p.avg_support;
+    out4[1] = p.min_support;
+    out4[2] = float(p.min_coverage);
+    out4[3] = float(p.max_coverage);
+    return p.isHQ ? 1 : 0;
+}
+
+// Classic anchor correction of the MSA built from the arguments
+// (src/msa.cpp:379-453); returns the high-quality flag.
+int orc_msa_corrected_anchor(int useQ, int la, const char* anchor, const char* anchorQ, int n, const char* cands,
+                             int candPitch, const char* candQ, int qualPitch, const int* lens, const int* shifts,
+                             const float* weights, float err, float cov, float mcov, char* out) {
+    const orc::MSA m = msaFrom(useQ, la, anchor, anchorQ, n, cands, candPitch, candQ, qualPitch, lens, shifts, weights);
+    const auto p = m.properties(err, cov, mcov);
+    const std::string c = m.correctedAnchor(p, err, cov, mcov);
+    std::memcpy(out, c.data(), std::size_t(la));
+    return p.isHQ ? 1 : 0;
+}
+
+// Different-region search of the MSA built from the arguments
+// (src/msa.cpp:532-734): returns 1 if a column was found, with the
+// candidates to remove flagged in remove[n].
+int orc_msa_different_region(int la, const char* anchor, int n, const char* cands, int candPitch, const int* lens,
+                             const int* shifts, const float* weights, int datasetCoverage, char* remove) {
+    const orc::MSA m = msaFrom(0, la, anchor, nullptr, n, cands, candPitch, nullptr, 0, lens, shifts, weights);
+    std::vector<char> rem;
+    const bool found = m.differentRegion(datasetCoverage, rem);
+    for (int i = 0; i < n; i++) remove[i] = found && !rem.empty() ? rem[std::size_t(i)] : 0;
+    return found ? 1 : 0;
+}
+
+float orc_se_filter_threshold(const float* ratios, int n, float err, float cov, float mcov) {
+    return orc::seFilterThreshold(ratios, n, err, cov, mcov);
+}
+
+int orc_candidate_in_range(int qb, int qe, int ab, int ae, int ncc, int forestOrGate) {
+    return orc::candidateInRange(qb, qe, ab, ae, ncc, forestOrGate != 0) ? 1 : 0;
+}
+
+float orc_overlap_weight(int nOps, int overlap, float maxMismatchRatio) {
+    return orc::overlapWeight(nOps, overlap, maxMismatchRatio);
+}
+
+// Edit list of a corrected read (include/corrector.hpp:1639-1761): the
+// number of edits (positions and bases written) if it is stored as edits,
+// -1 if it is stored as the whole sequence.
+int orc_make_edits(const char* corrected, const char* original, int len, int ambiguous, int* positions, char* bases) {
+    orc::Result r;
+    orc::makeEdits(r, std::string(corrected, std::size_t(len)), original, ambiguous != 0);
+    if (!r.useEdits) return -1;
+    for (std::size_t i = 0; i < r.edits.size(); i++) {
+        positions[i] = r.edits[i].first;
+        bases[i] = r.edits[i].second;
+    }
+    return int(r.edits.size());
+}
+
+// Forest vote (include/forest.hpp:217-226) of the .rf file at path for one
+// feature vector; kind 0 anchor (13 features), 1 candidates (14).  Returns
+// 1 accept, 0 reject, -1 if the file does not load.
+int orc_forest_decide(const char* path, int kind, std::uint32_t maxTrees, float threshold, const float* features) {
+    try {
+        orc::Forest f;
+        f.load(path, kind == 0 ? "13 extract_anchor_v3" : "14 extract_cands_v3", maxTrees, threshold);
+        return f.decide(features) ? 1 : 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// Combine rule of one read (src/correctionresultoutput.cpp:730-1023): n
+// results in stored order, flags bit 0 anchor, bit 1 high quality, each a
+// corrected sequence of the read's length (stored as edits or whole as the
+// edit rule decides); writes the output sequence, returns the label (0
+// uncorrected, 1 low-quality anchor alone, 2 agreeing results, 3 HQ anchor).
+int orc_combine(const char* original, int len, int ambiguous, int n, const int* flags, const char* corrected, char* out) {
+    std::vector<orc::Result> res(static_cast<std::size_t>(n));
+    std::vector<const orc::Result*> ptrs;
+    for (int i = 0; i < n; i++) {
+        res[i].isAnchor = (flags[i] & 1) != 0;
+        res[i].hq = (flags[i] & 2) != 0;
+        orc::makeEdits(res[i], std::string(corrected + std::size_t(i) * len, std::size_t(len)), original, ambiguous != 0);
+        ptrs.push_back(&res[i]);
+    }
+    std::string seq(original, std::size_t(len));
+    const auto c = orc::combine(ptrs, seq);
+    std::memcpy(out, seq.data(), seq.size());
+    return c.label;
+}
+
 int orc_tables_bucket(void* t, int map, std::uint64_t key, std::uint32_t* out, int cap) {
     auto* tb = static_cast<orc::Tables*>(t);
     auto it = tb->maps[map].index.find(key);

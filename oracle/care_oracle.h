@@ -27,6 +27,22 @@ void orc_tables_free(void* t);
 int orc_tables_query(void* t, const uint32_t* enc, int len, uint32_t self, uint32_t* out, int cap);
 int orc_tables_bucket(void* t, int map, uint64_t key, uint32_t* out, int cap);
 int orc_tables_query_limited(void* t, const uint32_t* enc, int len, uint32_t self, int limit, uint32_t* out, int cap);
+int orc_msa_build(int useQ, int la, const char* anchor, const char* anchorQ, int n, const char* cands, int candPitch,
+                  const char* candQ, int qualPitch, const int* lens, const int* shifts, const float* weights, int cap,
+                  int* dims, char* consensus, float* support, int* coverage, int* counts, float* wts, int* origCov);
+int orc_msa_properties(const float* support, const int* coverage, int ncols, int first, int last, float err, float cov,
+                       float mcov, float* out4);
+int orc_msa_corrected_anchor(int useQ, int la, const char* anchor, const char* anchorQ, int n, const char* cands,
+                             int candPitch, const char* candQ, int qualPitch, const int* lens, const int* shifts,
+                             const float* weights, float err, float cov, float mcov, char* out);
+int orc_msa_different_region(int la, const char* anchor, int n, const char* cands, int candPitch, const int* lens,
+                             const int* shifts, const float* weights, int datasetCoverage, char* remove);
+float orc_se_filter_threshold(const float* ratios, int n, float err, float cov, float mcov);
+int orc_candidate_in_range(int qb, int qe, int ab, int ae, int ncc, int forestOrGate);
+float orc_overlap_weight(int nOps, int overlap, float maxMismatchRatio);
+int orc_make_edits(const char* corrected, const char* original, int len, int ambiguous, int* positions, char* bases);
+int orc_forest_decide(const char* path, int kind, uint32_t maxTrees, float threshold, const float* features);
+int orc_combine(const char* original, int len, int ambiguous, int n, const int* flags, const char* corrected, char* out);
 #ifdef __cplusplus
 }
 #endif

@@ -111,7 +111,24 @@ inline std::vector<char>& lds() {
     return b;
 }
 inline char* dyn_lds() { return lds().data(); }
-inline int num_cus() { return 2; }
+// dynamic LDS bytes of the running launch
+inline size_t& dyn_lds_bytes() {
+    static size_t n = 0;
+    return n;
+}
+// EMU_CUS=<n>: compute units reported by the device attribute query
+// (default 2; 256 gives the launch grids of an MI355X)
+inline int num_cus() {
+    static const int v = [] {
+        const char* e = std::getenv("EMU_CUS");
+        const int n = e ? std::atoi(e) : 2;
+        return n > 0 ? n : 2;
+    }();
+    return v;
+}
+// kernel index checks (CARE_CHECK in the kernel sources, built with
+// CARE_EMU_CHECKS): a failed check aborts with its location
+inline void check(bool ok, const char* what, const char* file, int line);
 inline int lane() { return int(ctx.tid.x % kWaveSize); }
 inline Wave& my_wave() { return sched()->waves[ctx.tid.x / kWaveSize]; }
 
@@ -306,7 +323,8 @@ template <class K, class... Args>
 void launch_named(const char* name, K kernel, dim3 grid, dim3 block, size_t ldsBytes, Args... args) {
     if (block.x > block_cap()) block.x = block_cap();
     const unsigned nthreads = block.x;
-    lds().assign(ldsBytes + 16, 0);
+    lds().assign(ldsBytes ? ldsBytes : 1, 0);
+    dyn_lds_bytes() = ldsBytes;
     auto call = [&]() { kernel(args...); };
     Sched& s = sched_storage();
     s.body = [](void* p) { (*static_cast<decltype(call)*>(p))(); };
@@ -337,6 +355,13 @@ void launch_named(const char* name, K kernel, dim3 grid, dim3 block, size_t ldsB
 }
 
 }  // namespace emu
+
+inline void emu::check(bool ok, const char* what, const char* file, int line) {
+    if (ok) return;
+    std::fprintf(stderr, "emu: check failed: %s at %s:%d (block %u thread %u)\n", what, file, line, ctx.bid.x,
+                 ctx.tid.x);
+    std::abort();
+}
 
 #define threadIdx (emu::ctx.tid)
 #define blockIdx (emu::ctx.bid)

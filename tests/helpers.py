@@ -37,7 +37,8 @@ def simulate(case, directory):
 
 
 def oracle():
-    lib = ctypes.CDLL(ORACLE_LIB)
+    # CARE_ORACLE_LIB: another build of the oracle (tests/oracle_mutations.py)
+    lib = ctypes.CDLL(os.environ.get("CARE_ORACLE_LIB", ORACLE_LIB))
     u32p = ctypes.POINTER(ctypes.c_uint32)
     lib.orc_encode_2bit.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p]
     lib.orc_decode_2bit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]

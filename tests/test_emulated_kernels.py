@@ -81,3 +81,62 @@ def test_emulated_kernels_poisoned_memory_and_wave_order(emu_exe, env, tmp_path)
     case = SMALL["se_mixed_small"]
     want = _oracle(case, str(tmp_path))
     assert _run(emu_exe, case, str(tmp_path), env) == want
+
+
+# Kernel index checks (CARE_CHECK, every LDS region and global row index the
+# kernels write or gather) and the address / undefined-behaviour sanitizers,
+# on the launch grids of an MI355X (256 compute units) with small device
+# batches: the output must still be the oracle's.
+CHECKED_EXE = os.path.join(EMU_DIR, "_build", "checked", "care-gpu-emu")
+
+
+@pytest.fixture(scope="module")
+def checked_exe():
+    subprocess.run(["sh", os.path.join(EMU_DIR, "build_emu.sh"), "checked"], check=True)
+    return CHECKED_EXE
+
+
+@pytest.mark.parametrize("name", ["se_small", "se_mixed_small", "pe_small_q2"])
+def test_checked_kernels_match_oracle(checked_exe, name, tmp_path):
+    case = dict(SMALL[name])
+    case["args"] = case["args"] + ["--batchsize", "128"]
+    want = _oracle(dict(SMALL[name]), str(tmp_path))
+    env = {"EMU_CUS": "256", "ASAN_OPTIONS": "detect_leaks=0:detect_stack_use_after_return=0"}
+    assert _run(checked_exe, case, str(tmp_path), env) == want
+
+
+def test_padded_pitch_upload_matches_minimal_pitch(tmp_path):
+    """Rows wider than the minimal pitch are repacked on upload: results are
+    those of the minimal-pitch upload."""
+    import ctypes
+
+    import numpy as np
+
+    import care_amd
+    subprocess.run(["sh", os.path.join(EMU_DIR, "build_emu.sh")], check=True)
+    lib = care_amd._bind(ctypes.CDLL(os.path.join(EMU_DIR, "_build", "libcare_amd_emu.so")), care_amd.ENGINE_SYMBOLS)
+    case = SMALL["se_mixed_small"]
+    reads, _ = helpers.simulate(case, str(tmp_path))
+    rs = care_amd.ReadSet([reads], paired_end=False, qual_bits=8)
+    opt = care_amd.default_options(kmer_length=20, num_hash_maps=48, coverage=50.0, use_quality=1,
+                                   exclude_ambiguous=1, correct_candidates=1)
+    seqs, quals = rs.seqs, rs.quals
+    runs = []
+    for pad in (0, 5):
+        s = np.zeros((rs.n, rs.pitch + pad), dtype=np.uint32)
+        s[:, :rs.pitch] = seqs
+        s[:, rs.pitch:] = 0xDEADBEEF
+        q = np.zeros((rs.n, rs.qpitch + pad), dtype=np.uint32)
+        q[:, :rs.qpitch] = quals
+        q[:, rs.qpitch:] = 0xDEADBEEF
+        eng = care_amd.Engine(opt, device=0, lib=lib)
+        eng.upload(s, rs.lengths, rs.ambiguous, q, 8, rs.max_length)
+        eng.build()
+        ids, offs = eng.query(0, rs.n)
+        status, out, cids, cseqs = eng.correct(0, rs.n)
+        order = np.lexsort((np.arange(len(cids)), cids))
+        runs.append((ids.copy(), status.copy(), out.copy(), cids[order].copy(), cseqs[order].copy()))
+        eng.close()
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+    rs.close()
