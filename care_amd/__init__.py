@@ -146,6 +146,26 @@ ENGINE_SYMBOLS = {
                                                     ctypes.c_void_p, ctypes.c_uint64,
                                                     ctypes.POINTER(ctypes.c_uint64)]),
     "care_mlp_load_candidates": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "care_readstore_gather_sequences_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "care_readstore_gather_lengths_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                           ctypes.c_void_p, ctypes.c_void_p]),
+    "care_readstore_gather_qualities_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "care_readstore_ambiguous_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                      ctypes.c_void_p, ctypes.c_void_p]),
+    "care_minhash_max_candidates": (ctypes.c_int, [ctypes.c_void_p]),
+    "care_minhash_query_workspace_bytes": (ctypes.c_size_t, [ctypes.c_void_p, ctypes.c_int]),
+    "care_minhash_query_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "care_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "care_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "care_device_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                        ctypes.c_int]),
+    "care_stream_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "care_stream_destroy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "care_stream_synchronize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 HOST_SYMBOLS = {
@@ -403,6 +423,38 @@ class Engine:
         s = BatchStats()
         self._ck(self.lib.care_last_batch_stats(self.ctx, ctypes.byref(s)))
         return s
+
+    # ---- device buffers and streams (stream-ordered interface) ----
+    def device_array(self, nbytes):
+        """A device allocation of the context's device; freed by device_free."""
+        p = ctypes.c_void_p()
+        self._ck(self.lib.care_device_alloc(self.ctx, int(nbytes), ctypes.byref(p)))
+        return p.value
+
+    def device_free(self, p):
+        self._ck(self.lib.care_device_free(self.ctx, p))
+
+    def to_device(self, arr):
+        a = np.ascontiguousarray(arr)
+        p = self.device_array(max(1, a.nbytes))
+        self._ck(self.lib.care_device_copy(self.ctx, p, a.ctypes.data, a.nbytes, 1))
+        return p
+
+    def from_device(self, p, shape, dtype):
+        out = np.zeros(shape, dtype=dtype)
+        self._ck(self.lib.care_device_copy(self.ctx, out.ctypes.data, p, out.nbytes, 0))
+        return out
+
+    def stream_create(self):
+        s = ctypes.c_void_p()
+        self._ck(self.lib.care_stream_create(self.ctx, ctypes.byref(s)))
+        return s.value
+
+    def stream_destroy(self, s):
+        self._ck(self.lib.care_stream_destroy(self.ctx, s))
+
+    def stream_sync(self, s):
+        self._ck(self.lib.care_stream_synchronize(self.ctx, s))
 
     # ---- learned gate ----
     def mlp_load(self, params):

@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["api.hip", "minhash.hip", "correct.hip", "mlp.hip"]
+HIP_SOURCES = ["api.hip", "minhash.hip", "correct.hip", "mlp.hip", "store.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -491,6 +491,178 @@ int care_readstore_gather(care_context* ctx, const uint32_t* ids, int n, uint32_
     return CARE_OK;
 }
 
+// ---- stream-ordered device-buffer interface ------------------------------
+
+int care_readstore_gather_sequences_async(care_context* ctx, const uint32_t* d_ids, int n, uint32_t* d_out,
+                                          int out_pitch, struct ihipStream_t* stream) {
+    if (!ctx || (n > 0 && (!d_ids || !d_out))) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");
+    if (n < 0 || out_pitch < 1) return fail(CARE_ERR_ARGUMENT, "n must be >= 0 and out_pitch >= 1");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        launch_gather_sequences(ctx->rs, d_ids, n, d_out, out_pitch, stream);
+        HIP_TRY(hipGetLastError());
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_readstore_gather_lengths_async(care_context* ctx, const uint32_t* d_ids, int n, int32_t* d_out,
+                                        struct ihipStream_t* stream) {
+    if (!ctx || (n > 0 && (!d_ids || !d_out))) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");
+    if (n < 0) return fail(CARE_ERR_ARGUMENT, "n must be >= 0");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        launch_gather_lengths(ctx->rs, d_ids, n, d_out, stream);
+        HIP_TRY(hipGetLastError());
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_readstore_gather_qualities_async(care_context* ctx, const uint32_t* d_ids, int n, char* d_out, int out_pitch,
+                                          struct ihipStream_t* stream) {
+    if (!ctx || (n > 0 && (!d_ids || !d_out))) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");
+    if (!ctx->opt.use_quality) return fail(CARE_ERR_STATE, "the read store holds no quality scores (use_quality 0)");
+    if (n < 0 || out_pitch < 1) return fail(CARE_ERR_ARGUMENT, "n must be >= 0 and out_pitch >= 1");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        launch_gather_qualities(ctx->rs, d_ids, n, d_out, out_pitch, stream);
+        HIP_TRY(hipGetLastError());
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_readstore_ambiguous_async(care_context* ctx, const uint32_t* d_ids, int n, uint8_t* d_out,
+                                   struct ihipStream_t* stream) {
+    if (!ctx || (n > 0 && (!d_ids || !d_out))) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");
+    if (n < 0) return fail(CARE_ERR_ARGUMENT, "n must be >= 0");
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        launch_gather_ambiguous(ctx->rs, d_ids, n, d_out, stream);
+        HIP_TRY(hipGetLastError());
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
+int care_minhash_max_candidates(care_context* ctx) { return ctx ? ctx->maxCand : 0; }
+
+int care_device_alloc(care_context* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(CARE_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(CARE_ERR_DEVICE, "hipSetDevice failed");
+    const hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(CARE_ERR_DEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return CARE_OK;
+}
+
+int care_device_free(care_context* ctx, void* p) {
+    if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
+    if (!p) return CARE_OK;
+    (void)hipSetDevice(ctx->device);
+    const hipError_t e = hipFree(p);
+    if (e != hipSuccess) return fail(CARE_ERR_DEVICE, "hipFree: %s", hipGetErrorString(e));
+    return CARE_OK;
+}
+
+int care_device_copy(care_context* ctx, void* dst, const void* src, size_t bytes, int to_device) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(CARE_ERR_ARGUMENT, "null argument");
+    (void)hipSetDevice(ctx->device);
+    const hipError_t e = hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(CARE_ERR_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
+    return CARE_OK;
+}
+
+int care_stream_create(care_context* ctx, struct ihipStream_t** out) {
+    if (!ctx || !out) return fail(CARE_ERR_ARGUMENT, "null argument");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = nullptr;
+    const hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(CARE_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    *out = reinterpret_cast<struct ihipStream_t*>(s);
+    return CARE_OK;
+}
+
+int care_stream_destroy(care_context* ctx, struct ihipStream_t* stream) {
+    if (!ctx || !stream) return fail(CARE_ERR_ARGUMENT, "null argument");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(stream);
+    const hipError_t e = hipStreamDestroy(stream);
+    if (e != hipSuccess) return fail(CARE_ERR_DEVICE, "hipStreamDestroy: %s", hipGetErrorString(e));
+    return CARE_OK;
+}
+
+int care_stream_synchronize(care_context* ctx, struct ihipStream_t* stream) {
+    if (!ctx) return fail(CARE_ERR_ARGUMENT, "null context");
+    (void)hipSetDevice(ctx->device);
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return fail(CARE_ERR_DEVICE, "hipStreamSynchronize: %s", hipGetErrorString(e));
+    return CARE_OK;
+}
+
+namespace {
+// workspace of care_minhash_query_async: signature rows (8 h bytes per
+// anchor) and the query's spill rows for long hit lists (16 maxCand bytes
+// per anchor), 256-byte aligned parts
+size_t query_workspace_sig_bytes(const care_context* ctx, int n) {
+    return (size_t(n) * size_t(ctx->opt.num_hash_maps) * sizeof(u64) + 255) / 256 * 256;
+}
+size_t query_workspace_bytes(const care_context* ctx, int n) {
+    return query_workspace_sig_bytes(ctx, n) + size_t(n) * size_t(ctx->maxCand) * 4 * sizeof(u32);
+}
+}  // namespace
+
+size_t care_minhash_query_workspace_bytes(care_context* ctx, int n) {
+    return ctx && n > 0 ? query_workspace_bytes(ctx, n) : 0;
+}
+
+int care_minhash_query_async(care_context* ctx, const uint32_t* d_seqs, int seq_pitch, const int32_t* d_lengths,
+                             const uint32_t* d_read_ids, int n, uint32_t* d_candidates, int32_t* d_counts,
+                             void* d_workspace, size_t workspace_bytes, struct ihipStream_t* stream) {
+    if (!ctx || (n > 0 && (!d_seqs || !d_lengths || !d_candidates || !d_counts || !d_workspace)))
+        return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
+    if (n < 0 || seq_pitch < 1) return fail(CARE_ERR_ARGUMENT, "n must be >= 0 and seq_pitch >= 1");
+    if (n == 0) return CARE_OK;
+    if (reinterpret_cast<uintptr_t>(d_workspace) % 256 != 0) return fail(CARE_ERR_ARGUMENT, "workspace must be 256-byte aligned");
+    if (workspace_bytes < query_workspace_bytes(ctx, n))
+        return fail(CARE_ERR_CAPACITY, "workspace of %zu bytes, %zu needed", workspace_bytes, query_workspace_bytes(ctx, n));
+    const auto& o = ctx->opt;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        u64* sigs = static_cast<u64*>(d_workspace);
+        u32* spill = reinterpret_cast<u32*>(static_cast<char*>(d_workspace) + query_workspace_sig_bytes(ctx, n));
+        // the caller's sequences as a read store of n rows: lengths are cut to
+        // what the pitch holds
+        DevReadStore in{};
+        in.seqs = d_seqs;
+        in.lens = d_lengths;
+        in.n = u64(n);
+        in.pitch = seq_pitch;
+        in.maxLen = 16 * seq_pitch;
+        launch_signatures(in, 0, n, o.kmer_length, o.num_hash_maps, sigs, stream);
+        QueryAnchors qa;
+        qa.sigs = sigs;
+        qa.ids = d_read_ids;
+        qa.lens = d_lengths;
+        launch_query(ctx->rs, ctx->tables, 0, n, ctx->maxCand, ctx->queryCap, ctx->queryRegMax, o.exclude_ambiguous,
+                     o.correction_type == 3 ? o.candidate_limit : 0, d_candidates, d_counts, spill, nullptr, stream, qa);
+        HIP_TRY(hipGetLastError());
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}
+
 int care_minhash_signatures(care_context* ctx, uint64_t first, int n, uint64_t* out) {
     if (!ctx || !out) return fail(CARE_ERR_ARGUMENT, "null argument");
     if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");

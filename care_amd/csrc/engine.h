@@ -166,13 +166,28 @@ void launch_signatures(const DevReadStore& rs, u64 first, int n, int k, int h, u
 __host__ __device__ inline size_t query_wave_lds_bytes(int capL, int candLimit) {
     return (size_t(800) + 8 * size_t(capL) + (candLimit > 0 ? size_t(capL) : 0) + 15) / 16 * 16;
 }
+// Where a query takes its anchors from: by default anchor a is read
+// first + a of the read store with its stored signature row; a caller's own
+// sequences bring their signature rows (sigs, [n][h]), lengths, and the read
+// ids used to drop the anchor itself and, under exclude_ambiguous, ambiguous
+// anchors (ids may be null: no read is dropped as the anchor itself).
+struct QueryAnchors {
+    const u64* sigs = nullptr;
+    const u32* ids = nullptr;
+    const int* lens = nullptr;
+};
 // spill: 4 * maxCand words per anchor of the batch (sort/merge buffers of
 // anchors with more than capL hits); regMax: longest hit list sorted in
 // registers (0 = the kernel's maximum, 1024), longer lists are merged;
 // rawCount (may be null): table hits per anchor before de-duplication
 void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, int maxCand, int capL, int regMax,
                   int excludeAmbiguous, int candLimit, u32* candIds, int* candCount, u32* spill, int* rawCount,
-                  hipStream_t stream);
+                  hipStream_t stream, const QueryAnchors& anchors = QueryAnchors());
+// read-store gathers by read id into device buffers (store.hip)
+void launch_gather_sequences(const DevReadStore& rs, const u32* ids, int n, u32* out, int outPitch, hipStream_t s);
+void launch_gather_lengths(const DevReadStore& rs, const u32* ids, int n, int* out, hipStream_t s);
+void launch_gather_qualities(const DevReadStore& rs, const u32* ids, int n, char* out, int outPitch, hipStream_t s);
+void launch_gather_ambiguous(const DevReadStore& rs, const u32* ids, int n, u8* out, hipStream_t s);
 // totals[0] += sum(rawCount[0..n)), totals[1] += sum(candCount[0..n))
 void launch_sum_counts(const int* rawCount, const int* candCount, int n, unsigned long long* totals, hipStream_t s);
 // chunk: candidates staged per MSA pass (0 = automatic); gated: bit 0 the

@@ -87,7 +87,8 @@ __global__ void k_signatures(DevReadStore rs, u64 first, int n, int k, int h, u6
     u64* sig = partial + blockDim.x;
     for (u64 r = blockIdx.x; r < u64(n); r += gridDim.x) {
         const u64 id = first + r;
-        block_signature(rs.seqs + id * rs.pitch, rs.lens[id], k, h, canon, partial, sig);
+        // lengths beyond the LDS row (a caller's sequences) are cut to it
+        block_signature(rs.seqs + id * rs.pitch, min(rs.lens[id], rs.maxLen), k, h, canon, partial, sig);
         for (int j = threadIdx.x; j < h; j += blockDim.x) out[r * h + j] = sig[j];
         __syncthreads();
     }
@@ -609,7 +610,7 @@ __device__ __forceinline__ int keep_top(u32* out, HitBuf hits, int written, int 
 template <int kMaxK>
 __global__ void __launch_bounds__(256) k_query(DevReadStore rs, DevTables t, u64 first, int n, int maxCand, int capL,
                                                int regMax, int excludeAmbiguous, int candLimit, u32* candIds,
-                                               int* candCount, u32* spill, int* rawCount) {
+                                               int* candCount, u32* spill, int* rawCount, QueryAnchors qa) {
     HIP_DYNAMIC_SHARED(uint4, ldsRaw)
     const int wave = threadIdx.x / kWave;
     const int wpb = blockDim.x / kWave;
@@ -625,15 +626,19 @@ __global__ void __launch_bounds__(256) k_query(DevReadStore rs, DevTables t, u64
 #endif
 
     for (int a = blockIdx.x * wpb + wave; a < n; a += gridDim.x * wpb) {
-        const u64 id = first + u64(a);
+        // anchor read id (~0 for a caller's sequence without one), length
+        // and signature row
+        const u64 id = qa.ids ? u64(qa.ids[a]) : qa.sigs ? ~u64(0) : first + u64(a);
+        const bool stored = id < rs.n;
         // the lane index is made opaque in every iteration: otherwise the
         // compiler hoists the lane-dependent addresses of the sort and merge
         // networks out of this loop and keeps them all in registers (about
         // 190 instead of 104 VGPRs for the 32-per-lane sort)
         int lane = lane_id();
         asm volatile("" : "+v"(lane));
-        const int len = rs.lens[id];
-        const bool skip = (excludeAmbiguous && rs.ambig[id]) || len < t.k;
+        const int len = qa.lens ? qa.lens[a] : rs.lens[id];
+        const u64* sig = qa.sigs ? qa.sigs + u64(a) * u64(h) : t.sigs + id * u64(h);
+        const bool skip = (excludeAmbiguous && stored && rs.ambig[id]) || len < t.k;
         if (skip) {
             if (lane == 0) {
                 candCount[a] = 0;
@@ -646,7 +651,7 @@ __global__ void __launch_bounds__(256) k_query(DevReadStore rs, DevTables t, u64
         const u32* list = nullptr;
         if (lane < h) {
             const DevMap m = t.maps[lane];
-            const u64 slot = find_slot(m.keys, m.mask, t.sigs[id * u64(h) + lane]);
+            const u64 slot = find_slot(m.keys, m.mask, sig[lane]);
             if (slot != ~u64(0)) {
                 const uint2 oc = m.offcnt[slot];
                 c = int(oc.y);
@@ -659,7 +664,7 @@ __global__ void __launch_bounds__(256) k_query(DevReadStore rs, DevTables t, u64
             if (lane >= o) incl += v;
         }
         const int total = __shfl(incl, kWave - 1, kWave);
-        CARE_CHECK(total <= maxCand && id < rs.n);
+        CARE_CHECK(total <= maxCand && (stored || qa.sigs));
         if (lane < h) {
             counts[lane] = incl - c;
             lists[lane] = list;
@@ -734,7 +739,7 @@ int query_lds_bytes(int h, int cap, int maxLen, int threads, int candLimit, int 
 
 void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, int maxCand, int capL, int regMax,
                   int excludeAmbiguous, int candLimit, u32* candIds, int* candCount, u32* spill, int* rawCount,
-                  hipStream_t stream) {
+                  hipStream_t stream, const QueryAnchors& qa) {
     if (n <= 0) return;
     const int threads = 256;
     const int wpb = threads / kWave;
@@ -748,10 +753,10 @@ void launch_query(const DevReadStore& rs, const DevTables& t, u64 first, int n, 
     // used only when asked for lists of more than 1024 hits
     if (regMax > 16 * kWave)
         hipLaunchKernelGGL(k_query<32>, dim3(blocks), dim3(threads), lds, stream, rs, t, first, n, maxCand, capL, regMax,
-                           excludeAmbiguous, candLimit, candIds, candCount, spill, rawCount);
+                           excludeAmbiguous, candLimit, candIds, candCount, spill, rawCount, qa);
     else
         hipLaunchKernelGGL(k_query<16>, dim3(blocks), dim3(threads), lds, stream, rs, t, first, n, maxCand, capL, regMax,
-                           excludeAmbiguous, candLimit, candIds, candCount, spill, rawCount);
+                           excludeAmbiguous, candLimit, candIds, candCount, spill, rawCount, qa);
 }
 
 void launch_sum_counts(const int* rawCount, const int* candCount, int n, unsigned long long* totals, hipStream_t s) {

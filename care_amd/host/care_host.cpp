@@ -310,6 +310,8 @@ struct HostOptions {
     int batchsize = 0;
     bool autotune = true;  // time launch shapes on a sample of anchors before STEP 2
     std::string forestAnchor, forestCands;
+    // checkpoint files of the reference (preprocessed reads, hash tables)
+    std::string saveReads, loadReads, saveTables, loadTables;
     std::string gateFile;       // learned gate parameters (correctionType 3)
     std::string gateFileCands;  // candidate gate parameters (correctionTypeCands 3; default: the anchor gate)
     uint32_t maxTreesAnchor = 0xFFFFFFFFu, maxTreesCands = 0xFFFFFFFFu;
@@ -398,10 +400,14 @@ HostOptions parseOptions(int argc, char** argv) {
         else if (a == "--gzoutput") o.gzoutput = flag();
         else if (a == "-p" || a == "--showProgress") o.showProgress = flag();
         else if (a == "--enforceHashmapCount") (void)flag();
+        else if (a == "--save-preprocessedreads-to") o.saveReads = value();
+        else if (a == "--load-preprocessedreads-from") o.loadReads = value();
+        else if (a == "--save-hashtables-to") o.saveTables = value();
+        else if (a == "--load-hashtables-from") o.loadTables = value();
+        // memory limits, the temporary directory and the reference's table
+        // implementation choices do not apply to this build's device tables
         else if (a == "-m" || a == "--memTotal" || a == "--memHashtables" || a == "--tempdir" || a == "--hashloadfactor" ||
-                 a == "--warpcore" || a == "--replicateGpuData" || a == "--save-preprocessedreads-to" ||
-                 a == "--load-preprocessedreads-from" || a == "--save-hashtables-to" || a == "--load-hashtables-from" ||
-                 a == "--singlehash" ||
+                 a == "--warpcore" || a == "--replicateGpuData" || a == "--singlehash" ||
                  a == "--samplingRateAnchor" || a == "--samplingRateCands")
             (void)value();
         else throw std::runtime_error("unknown option " + a);
@@ -565,6 +571,10 @@ int runMain(int argc, char** argv) {
         return 0;
     }
     if (o.gpus.size() > 1) std::fprintf(stderr, "Using only GPU %d of the given list (single-device build)\n", o.gpus[0]);
+    if (!o.saveReads.empty() || !o.loadReads.empty() || !o.saveTables.empty() || !o.loadTables.empty())
+        throw std::runtime_error(
+            "--save-preprocessedreads-to, --load-preprocessedreads-from, --save-hashtables-to and "
+            "--load-hashtables-from are not supported by this build");
     if (o.outputs.size() != 1 && o.outputs.size() != o.inputs.size())
         throw std::runtime_error("number of output files must be 1 or equal to the number of input files");
     if (o.eng.paired_end && o.inputs.size() > 2) throw std::runtime_error("paired-end input takes one or two files");

@@ -86,6 +86,66 @@ int care_readstore_upload(care_context* ctx, uint64_t num_reads, int max_length,
 int care_readstore_gather(care_context* ctx, const uint32_t* ids, int n, uint32_t* out_seqs, int out_pitch,
                           int32_t* out_lengths);
 
+/* ---- Stream-ordered interface on device buffers -------------------------
+ * The calls below take device pointers and a HIP stream (hipStream_t is
+ * `struct ihipStream_t*`, hip/hip_runtime_api.h; NULL is the null stream),
+ * enqueue their kernels on it and return at once: outputs are valid when
+ * the stream reaches that point.  They allocate nothing and never
+ * synchronise the host.  Ids outside the read store give zero rows. */
+struct ihipStream_t;
+
+/* Rows of read ids d_ids[0..n) into d_out (out_pitch words per row; words
+ * past the store's pitch are zero).  Replaces GpuReadStorage::gatherSequences
+ * (include/gpu/gpureadstorage.cuh:22-126; MultiGpuReadStorage
+ * include/gpu/multigpureadstorage.cuh:669). */
+int care_readstore_gather_sequences_async(care_context* ctx, const uint32_t* d_ids, int n, uint32_t* d_out,
+                                          int out_pitch, struct ihipStream_t* stream);
+/* Lengths of read ids d_ids[0..n).  Replaces
+ * GpuReadStorage::gatherSequenceLengths (include/gpu/multigpureadstorage.cuh:1341). */
+int care_readstore_gather_lengths_async(care_context* ctx, const uint32_t* d_ids, int n, int32_t* d_out,
+                                        struct ihipStream_t* stream);
+/* Quality characters of read ids d_ids[0..n) (out_pitch chars per row;
+ * binned qualities decode to their bins' characters).  Needs use_quality.
+ * Replaces GpuReadStorage::gatherQualities (include/gpu/multigpureadstorage.cuh:1040). */
+int care_readstore_gather_qualities_async(care_context* ctx, const uint32_t* d_ids, int n, char* d_out,
+                                          int out_pitch, struct ihipStream_t* stream);
+/* 1 for reads whose input held a non-ACGT base.  Replaces
+ * GpuReadStorage::areSequencesAmbiguous (include/gpu/multigpureadstorage.cuh:637). */
+int care_readstore_ambiguous_async(care_context* ctx, const uint32_t* d_ids, int n, uint8_t* d_out,
+                                   struct ihipStream_t* stream);
+
+/* Stride of the candidate lists written by care_minhash_query_async:
+ * num_hash_maps x the per-map value limit (include/minhasherlimit.hpp:10-15). */
+int care_minhash_max_candidates(care_context* ctx);
+/* Device workspace bytes care_minhash_query_async needs for n sequences. */
+size_t care_minhash_query_workspace_bytes(care_context* ctx, int n);
+/* Candidates of n sequences given on the device (2-bit rows of seq_pitch
+ * words, lengths up to 16 * seq_pitch): their minhash signatures, the table
+ * hits of every map, sorted and de-duplicated, without the read d_read_ids[i]
+ * itself and, under exclude_ambiguous, without ambiguous reads (an
+ * ambiguous d_read_ids[i] gets no candidates; d_read_ids may be NULL).
+ * Sequence i's candidates go to d_candidates[i * care_minhash_max_candidates
+ * ..], their number to d_counts[i].  d_workspace: 256-byte aligned,
+ * care_minhash_query_workspace_bytes(ctx, n) bytes, not used by another call
+ * in flight.  Replaces GpuMinhasher::determineNumValues + retrieveValues
+ * (include/gpu/gpuminhasher.cuh:20-110) followed by
+ * GpuMinhashQueryFilter::keepDistinctAndNotMatching
+ * (include/gpu/minhashqueryfilter.cuh:21-100). */
+int care_minhash_query_async(care_context* ctx, const uint32_t* d_seqs, int seq_pitch, const int32_t* d_lengths,
+                             const uint32_t* d_read_ids, int n, uint32_t* d_candidates, int32_t* d_counts,
+                             void* d_workspace, size_t workspace_bytes, struct ihipStream_t* stream);
+
+/* Device memory and streams of the context's device, for callers without a
+ * HIP binding of their own (no counterpart in the reference, whose callers
+ * hold rmm resources and CUDA streams).  to_device: 1 host to device, 0
+ * device to host; copies are synchronous. */
+int care_device_alloc(care_context* ctx, size_t bytes, void** out);
+int care_device_free(care_context* ctx, void* p);
+int care_device_copy(care_context* ctx, void* dst, const void* src, size_t bytes, int to_device);
+int care_stream_create(care_context* ctx, struct ihipStream_t** out);
+int care_stream_destroy(care_context* ctx, struct ihipStream_t* stream);
+int care_stream_synchronize(care_context* ctx, struct ihipStream_t* stream);
+
 /* Builds all minhash tables on the device from the uploaded read store.
  * Replaces constructGpuMinhasherFromGpuReadStorage / GpuMinhasher::insert +
  * compact (src/gpu/gpuminhasherconstruction.cu:37-253,

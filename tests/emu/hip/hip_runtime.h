@@ -530,11 +530,15 @@ inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* nb, K, int, 
     *nb = int(byLds < 8 ? (byLds ? byLds : 1) : 8);
     return hipSuccess;
 }
+// device allocations are 256-byte aligned, as on the device
 template <class T>
 inline hipError_t hipMalloc(T** p, size_t bytes) {
-    *p = static_cast<T*>(std::calloc(1, bytes ? bytes : 1));
-    if (*p && emu::poison_byte() >= 0) std::memset(static_cast<void*>(*p), emu::poison_byte(), bytes);
-    return *p ? hipSuccess : hipErrorInvalidValue;
+    const size_t n = (bytes + 255) / 256 * 256 + 256 * (bytes == 0);
+    void* v = std::aligned_alloc(256, n);
+    if (!v) return hipErrorInvalidValue;
+    std::memset(v, emu::poison_byte() >= 0 ? emu::poison_byte() : 0, n);
+    *p = static_cast<T*>(v);
+    return hipSuccess;
 }
 inline hipError_t hipFree(void* p) {
     std::free(p);
