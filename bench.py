@@ -126,6 +126,7 @@ def cpu_baseline(reads, w, sample, threads):
                     break
     except OSError:
         pass
+    cpu_baseline.output = os.path.join(outdir, "oracle.fastq")
     return {"value": float(m.group(2)), "unit": "reads/s", "cores": threads, "kind": "port",
             "sample": "first %d anchors of the same read set (tables built over all reads, not timed), "
                       "query+correction %.2f s on %d threads, %s" % (sample, float(m.group(1)), threads, model)}
@@ -200,15 +201,57 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=Tru
             "corrected": int(st.corrected)}
 
 
+def step_times(stdout):
+    """STEP1 / STEP2 wall seconds printed by care-gpu."""
+    t = {}
+    for k in ("STEP1", "STEP2"):
+        m = re.search(r"# elapsed time \(%s\): ([0-9.]+)s" % k, stdout)
+        if m:
+            t[k.lower() + "_s"] = float(m.group(1))
+    return t
+
+
 def quality(care_amd, reads, truth, w, extra, outdir):
-    """Base-level TP/FP/FN of the care-gpu command line's output."""
+    """Base-level TP/FP/FN of the care-gpu command line's output, with its
+    STEP1 / STEP2 wall times (STEP2: the correction of every read including
+    the host's edit lists)."""
     os.makedirs(outdir, exist_ok=True)
     cmd = [care_amd.CARE_GPU, "-i", reads, "-o", "c.fastq", "-d", outdir, "-g", "0", "-t", "16"] + w["args"] + extra
     p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if p.returncode != 0:
         raise RuntimeError(p.stdout[-2000:])
     r = care_amd.evaluate(reads, os.path.join(outdir, "c.fastq"), truth)
-    return {k: r[k] for k in ("TP", "FP", "FN")}
+    out = {k: r[k] for k in ("TP", "FP", "FN")}
+    out.update(step_times(p.stdout))
+    return out
+
+
+def fastq_records(path):
+    with open(path, "rb") as f:
+        while True:
+            h = f.readline()
+            if not h:
+                return
+            s = f.readline()
+            f.readline()
+            f.readline()
+            yield s
+
+
+def parity(care_amd, reads, w, sample, oracle_out, outdir):
+    """Reads whose corrected sequence differs between care-gpu (plain path)
+    and the oracle's output on the same first `sample` anchors."""
+    os.makedirs(outdir, exist_ok=True)
+    cmd = [care_amd.CARE_GPU, "-i", reads, "-o", "p.fastq", "-d", outdir, "-g", "0", "-t", "16",
+           "--fixedNumberOfReads", str(sample)] + w["args"]
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(p.stdout[-2000:])
+    diff = total = 0
+    for a, b in zip(fastq_records(os.path.join(outdir, "p.fastq")), fastq_records(oracle_out)):
+        total += 1
+        diff += a != b
+    return {"anchors": sample, "reads_compared": total, "differing_reads": diff, "against": "oracle (care-oracle)"}
 
 
 def main():
@@ -275,7 +318,16 @@ def main():
                                     os.path.join(work, "gated"))
         log("bench: quality %s" % json.dumps(qual))
 
-    head = gated if gated is not None else plain
+    # the gated path is the headline only where it meets the target's
+    # quality rule: false positives no higher, true positives within 1 %
+    rule = None
+    if gated is not None and qual is not None and "gated" in qual:
+        gq, pq = qual["gated"], qual["plain"]
+        rule = {"fp_not_higher": gq["FP"] <= pq["FP"], "tp_within_1pct": gq["TP"] >= 0.99 * pq["TP"],
+                "tp_change_pct": round(100.0 * (gq["TP"] - pq["TP"]) / max(1, pq["TP"]), 3),
+                "fp_change_pct": round(100.0 * (gq["FP"] - pq["FP"]) / max(1, pq["FP"]), 3)}
+        rule["met"] = rule["fp_not_higher"] and rule["tp_within_1pct"]
+    head = gated if gated is not None and (rule is None or rule["met"]) else plain
     value = n / (head["ms"] / 1e3)
 
     # algorithmic bytes of the fused correction kernel per step: anchor and
@@ -288,7 +340,7 @@ def main():
         alg = n * (pitch_bytes + w["max_length"] + 1) + r["candidates"] * (pitch_bytes + 4 + 16 + qbytes)
         launch_ms = r["correct_ms"] / r["launches"]
         achieved = (alg / r["launches"]) / (launch_ms / 1e3) / 1e9
-        kernel = "k_correct<%s,false>" % ("true" if r is gated else "false")
+        kernel = "k_correct<%s,false>" % ("true" if r is gated else "false")  # noqa
         return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kernel, args.workload),
                 "kernel": kernel, "launches_per_step": r["launches"],
@@ -306,9 +358,15 @@ def main():
                 "bytes_per_launch": int(alg / r["launches"]), "avg_launch_ms": round(launch_ms, 4)}
 
     cpu = None
+    par = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(reads, w, min(args.cpu_sample, n), threads)
+        sample = min(args.cpu_sample, n)
+        cpu = cpu_baseline(reads, w, sample, threads)
+        if cpu is not None and w.get("parity", True):
+            par = parity(care_amd, reads, w, sample, cpu_baseline.output,
+                         os.path.join(os.path.dirname(reads), "parity"))
+            log("bench: parity %s" % json.dumps(par))
 
     def summary(r):
         return {"reads_per_s": round(n / (r["ms"] / 1e3), 1), "ms_per_step": round(r["ms"], 3),
@@ -326,6 +384,11 @@ def main():
         config[name] = summary(r)
     if qual is not None:
         config["quality"] = qual
+    if rule is not None:
+        config["gated_quality_rule"] = rule
+    config["headline_path"] = "mlp_gated" if head is gated else "plain"
+    if par is not None:
+        config["parity"] = par
     line = {
         "metric": "reads corrected/s on 1 MI355X: plain vs MLP-gated at equal base-level TP/FP"
                   if gated is not None else "reads corrected/s on 1 MI355X (plain classic path)",

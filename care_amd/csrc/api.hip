@@ -122,6 +122,7 @@ struct care_context {
     hipStream_t copyStream = nullptr;
     hipEvent_t copyDone = nullptr;
     int numCUs = 0;
+    size_t freeAtCreate = 0;  // device memory free when the context was created
 
     // read store
     DevBuf<u32> seqs;
@@ -218,12 +219,17 @@ void ensure_batch(care_context* ctx, int batch) {
     ctx->batchCap = batch;
 }
 
+// Anchors per device batch: the batch scratch (candidate ids and two
+// records per candidate slot, h x cap slots per anchor) gets a quarter of the
+// device memory free when the context was created, between 2 and 24 GiB, and
+// a batch holds at most 2^18 anchors.  Fewer, larger batches mean fewer
+// launches and fewer drains of the persistent correction kernel.
 int default_batch(care_context* ctx) {
     if (ctx->requestedBatch > 0) return ctx->requestedBatch;
-    // bound scratch to ~2 GiB: candidate ids + two metadata copies per slot
     const size_t perAnchor = size_t(ctx->maxCand) * (sizeof(u32) + 2 * sizeof(CandMeta)) + ctx->rs.maxLen + 64;
-    size_t b = (size_t(2) << 30) / perAnchor;
-    b = std::max<size_t>(256, std::min<size_t>(b, 65536));
+    const size_t budget = std::min(std::max(ctx->freeAtCreate / 4, size_t(2) << 30), size_t(24) << 30);
+    size_t b = budget / perAnchor;
+    b = std::max<size_t>(256, std::min<size_t>(b, size_t(1) << 18));
     return int(b & ~size_t(1));
 }
 
@@ -295,6 +301,9 @@ int care_context_create(int device, const care_options* opt, care_context** out)
         HIP_TRY(hipEventCreateWithFlags(&ctx->copyDone, hipEventDisableTiming));
         for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipDeviceGetAttribute(&ctx->numCUs, hipDeviceAttributeMultiprocessorCount, device));
+        size_t freeB = 0, totalB = 0;
+        HIP_TRY(hipMemGetInfo(&freeB, &totalB));
+        ctx->freeAtCreate = freeB;
         ctx->cap = resultsPerMapThreshold(opt->coverage);
         ctx->maxCand = opt->num_hash_maps * ctx->cap;
         ctx->queryCap = std::min((ctx->maxCand + 15) / 16 * 16, kDefaultQueryCap);

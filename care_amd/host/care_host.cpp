@@ -20,11 +20,13 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <sys/stat.h>
@@ -309,6 +311,7 @@ struct HostOptions {
     std::vector<int> gpus;
     int batchsize = 0;
     bool autotune = true;  // time launch shapes on a sample of anchors before STEP 2
+    int candResultsPerAnchor = 64;  // room for candidate corrections per anchor of a chunk
     std::string forestAnchor, forestCands;
     // checkpoint files of the reference (preprocessed reads, hash tables)
     std::string saveReads, loadReads, saveTables, loadTables;
@@ -396,6 +399,10 @@ HostOptions parseOptions(int argc, char** argv) {
             if (v > 0) o.maxTreesCands = uint32_t(v);
         }
         else if (a == "--batchsize") o.batchsize = std::stoi(value());
+        else if (a == "--candidateResultsPerAnchor") {
+            o.candResultsPerAnchor = std::stoi(value());
+            if (o.candResultsPerAnchor < 1) throw std::runtime_error("--candidateResultsPerAnchor must be >= 1");
+        }
         else if (a == "--autotune") o.autotune = flag();  // --autotune=false keeps the default shapes
         else if (a == "--gzoutput") o.gzoutput = flag();
         else if (a == "-p" || a == "--showProgress") o.showProgress = flag();
@@ -478,6 +485,31 @@ struct ResultStore {
         dataLen.push_back(cnt);
     }
 };
+
+// One store holding the entries of `parts` in order (data offsets shifted).
+ResultStore mergeStores(std::vector<ResultStore>& parts) {
+    ResultStore all;
+    size_t n = 0, bytes = 0;
+    for (const auto& p : parts) {
+        n += p.readId.size();
+        bytes += p.data.size();
+    }
+    all.readId.reserve(n);
+    all.flags.reserve(n);
+    all.dataOff.reserve(n);
+    all.dataLen.reserve(n);
+    all.data.reserve(bytes);
+    for (auto& p : parts) {
+        const u64 base = all.data.size();
+        all.readId.insert(all.readId.end(), p.readId.begin(), p.readId.end());
+        all.flags.insert(all.flags.end(), p.flags.begin(), p.flags.end());
+        all.dataLen.insert(all.dataLen.end(), p.dataLen.begin(), p.dataLen.end());
+        for (u64 off : p.dataOff) all.dataOff.push_back(off + base);
+        all.data.insert(all.data.end(), p.data.begin(), p.data.end());
+        p = ResultStore();
+    }
+    return all;
+}
 
 // src/correctionresultoutput.cpp:730-1023 (combineMultipleCorrectionResults1_rawtcs2);
 // returns label 0 uncorrected, 1 lq anchor only, 2 lq with candidates, 3 hq
@@ -620,62 +652,160 @@ int runMain(int argc, char** argv) {
     check(care_readstore_upload(ctx, rs->n, rs->maxLen, rs->seqs.data(), rs->pitch, rs->lens.data(), rs->ambig.data(),
                                 o.eng.use_quality ? rs->quals.data() : nullptr, rs->qpitch, rs->qbits));
     check(care_minhash_build(ctx));
-    const double t1 = now_s();
-    std::printf("# elapsed time (STEP1): %.6fs\n", t1 - t0);
+    std::printf("# elapsed time (STEP1): %.6fs\n", now_s() - t0);
+
+    // launch shapes timed on the first anchors (set-up, like the table
+    // build; results do not depend on them)
+    const u64 numAnchors = o.fixedNumberOfReads ? std::min<u64>(o.fixedNumberOfReads, rs->n) : rs->n;
+    if (o.autotune && numAnchors >= 4096) {
+        const double ta = now_s();
+        care_tuning t{};
+        check(care_autotune(ctx, 0, int(std::min<u64>(numAnchors, 65536) & ~u64(1)), &t));
+        std::printf("Launch shapes: query %d hits in LDS, correction %d threads x %d per CU, MSA chunk %d "
+                    "(tuned in %.3fs)\n",
+                    t.query_lds_hits, t.correct_threads, t.correct_blocks_per_cu, t.msa_chunk, now_s() - ta);
+    }
+    // Result buffers in page-locked memory (the engine copies into them by
+    // DMA), two sets: the device fills one while host threads turn the
+    // other's corrections into edit lists.
+    const int pitch = (rs->maxLen + 15) / 16 * 16;
+    const int chunk = int(std::min<u64>(u64(1) << 17, std::max<u64>(numAnchors + (numAnchors & 1), 2)));
+    const int candCap = o.eng.correct_candidates ? chunk * o.candResultsPerAnchor : 0;
+    struct Slot {
+        PinnedArray<u8> status;
+        PinnedArray<char> seqs;
+        PinnedArray<u32> candIds;
+        PinnedArray<char> candSeqs;
+        u64 first = 0;
+        int n = 0;
+        int numCands = 0;
+        Slot(size_t chunkN, size_t capN, size_t pitchN)
+            : status(chunkN), seqs(chunkN * pitchN), candIds(std::max<size_t>(1, capN)),
+              candSeqs(std::max<size_t>(1, capN) * pitchN) {}
+    };
+    std::vector<std::unique_ptr<Slot>> slots;
+    for (int i = 0; i < 2; i++) slots.push_back(std::make_unique<Slot>(size_t(chunk), size_t(candCap), size_t(pitch)));
 
     // STEP 2
     std::printf("STEP 2: Error correction\n");
-    const u64 numAnchors = o.fixedNumberOfReads ? std::min<u64>(o.fixedNumberOfReads, rs->n) : rs->n;
-    if (o.autotune && numAnchors >= 4096) {
-        // launch shapes timed on the first anchors; results do not depend on them
-        care_tuning t{};
-        check(care_autotune(ctx, 0, int(std::min<u64>(numAnchors, 65536) & ~u64(1)), &t));
-        std::printf("Launch shapes: query %d hits in LDS, correction %d threads x %d per CU, MSA chunk %d\n",
-                    t.query_lds_hits, t.correct_threads, t.correct_blocks_per_cu, t.msa_chunk);
-    }
-    const int pitch = (rs->maxLen + 15) / 16 * 16;
-    const int chunk = 1 << 18;
-    // result buffers in page-locked memory: the engine copies into them by DMA
-    const int candCap = o.eng.correct_candidates ? chunk * 64 : 0;
-    PinnedArray<u8> status{static_cast<size_t>(chunk)};
-    PinnedArray<char> seqs{static_cast<size_t>(chunk) * pitch};
-    PinnedArray<u32> candIds{static_cast<size_t>(std::max(1, candCap))};
-    PinnedArray<char> candSeqs{static_cast<size_t>(std::max(1, candCap)) * pitch};
-    ResultStore anchors, cands;
+    const double t1 = now_s();
     std::vector<u8> hqAnchor(rs->n, 0);
-    std::string original;
+    std::vector<ResultStore> anchorParts(static_cast<size_t>(threads)), candParts(static_cast<size_t>(threads));
     auto decode = [&](u64 id, std::string& out) {
         const int len = rs->lens[id];
         out.resize(size_t(len));
         const u32* row = rs->seqs.data() + id * rs->pitch;
         for (int p = 0; p < len; p++) out[size_t(p)] = "ACGT"[(row[p >> 4] >> (30 - 2 * (p & 15))) & 3u];
     };
-    for (u64 first = 0; first < numAnchors; first += chunk) {
-        int n = int(std::min<u64>(chunk, numAnchors - first));
-        if (o.eng.paired_end && (n & 1)) n++;  // whole pairs
-        if (first + u64(n) > rs->n) n = int(rs->n - first);
-        care_batch_result r{};
-        r.anchor_status = status.data();
-        r.anchor_sequences = seqs.data();
-        r.sequence_pitch = pitch;
-        r.candidate_read_ids = candIds.data();
-        r.candidate_sequences = candSeqs.data();
-        r.candidate_capacity = candCap;
-        check(care_correct_batch(ctx, first, n, &r));
-        for (int i = 0; i < n; i++) {
-            if (!(status[size_t(i)] & 1)) continue;
-            const u64 id = first + u64(i);
-            const bool hq = (status[size_t(i)] & 2) != 0;
-            if (hq) hqAnchor[id] = 1;
-            decode(id, original);
-            anchors.add(u32(id), true, hq, seqs.data() + size_t(i) * pitch, original.data(), rs->lens[id], rs->ambig[id] != 0);
+    // edit lists of one filled slot, anchors and candidate results split
+    // over the host threads (each thread appends to its own stores)
+    auto processSlot = [&](Slot& sl) {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; t++) {
+            pool.emplace_back([&, t]() {
+                std::string original;
+                const int a0 = int(u64(sl.n) * u64(t) / u64(threads)), a1 = int(u64(sl.n) * u64(t + 1) / u64(threads));
+                for (int i = a0; i < a1; i++) {
+                    if (!(sl.status[size_t(i)] & 1)) continue;
+                    const u64 id = sl.first + u64(i);
+                    const bool hq = (sl.status[size_t(i)] & 2) != 0;
+                    if (hq) hqAnchor[id] = 1;
+                    decode(id, original);
+                    anchorParts[size_t(t)].add(u32(id), true, hq, sl.seqs.data() + size_t(i) * pitch, original.data(),
+                                               rs->lens[id], rs->ambig[id] != 0);
+                }
+                const int c0 = int(u64(sl.numCands) * u64(t) / u64(threads));
+                const int c1 = int(u64(sl.numCands) * u64(t + 1) / u64(threads));
+                for (int i = c0; i < c1; i++) {
+                    const u32 id = sl.candIds[size_t(i)];
+                    decode(id, original);
+                    candParts[size_t(t)].add(id, false, false, sl.candSeqs.data() + size_t(i) * pitch, original.data(),
+                                             rs->lens[id], rs->ambig[id] != 0);
+                }
+            });
         }
-        for (int i = 0; i < r.num_candidates; i++) {
-            const u32 id = candIds[size_t(i)];
-            decode(id, original);
-            cands.add(id, false, false, candSeqs.data() + size_t(i) * pitch, original.data(), rs->lens[id], rs->ambig[id] != 0);
+        for (auto& th : pool) th.join();
+    };
+    // The device thread corrects chunk after chunk into the two slots; the
+    // host side processes each slot as soon as it is filled.  A chunk whose
+    // candidate results exceed the slot's capacity is corrected again in
+    // halves.
+    std::mutex mu;
+    std::condition_variable cv;
+    int filled[2] = {0, 0};  // 1: results waiting for the host
+    bool deviceDone = false;
+    std::string deviceError;
+    std::vector<int> order;  // slot of every filled batch, in order
+    std::thread device([&]() {
+        // pending anchor ranges as a stack, the next range on top
+        std::vector<std::pair<u64, int>> work;
+        for (u64 first = 0; first < numAnchors; first += u64(chunk)) {
+            int n = int(std::min<u64>(u64(chunk), numAnchors - first));
+            if (o.eng.paired_end && (n & 1)) n++;  // whole pairs
+            if (first + u64(n) > rs->n) n = int(rs->n - first);
+            work.emplace_back(first, n);
         }
+        std::reverse(work.begin(), work.end());
+        int next = 0;
+        while (!work.empty()) {
+            const auto [first, n] = work.back();
+            work.pop_back();
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return !filled[next]; });
+            }
+            Slot& sl = *slots[size_t(next)];
+            care_batch_result r{};
+            r.anchor_status = sl.status.data();
+            r.anchor_sequences = sl.seqs.data();
+            r.sequence_pitch = pitch;
+            r.candidate_read_ids = sl.candIds.data();
+            r.candidate_sequences = sl.candSeqs.data();
+            r.candidate_capacity = candCap;
+            const int rc = care_correct_batch(ctx, first, n, &r);
+            if (rc == CARE_ERR_CAPACITY && n > 2) {
+                // candidate results beyond the slot: the two halves separately
+                std::printf("Candidate corrections of reads %llu..%llu exceed the result buffer: corrected in halves\n",
+                            (unsigned long long)first, (unsigned long long)(first + u64(n) - 1));
+                int h = n / 2;
+                if (o.eng.paired_end) h &= ~1;
+                work.emplace_back(first + u64(h), n - h);
+                work.emplace_back(first, h);
+                continue;
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            if (rc != CARE_OK) {
+                deviceError = care_last_error();
+                break;
+            }
+            sl.first = first;
+            sl.n = n;
+            sl.numCands = r.num_candidates;
+            filled[next] = 1;
+            order.push_back(next);
+            cv.notify_all();
+            next ^= 1;
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        deviceDone = true;
+        cv.notify_all();
+    });
+    for (size_t k = 0;; k++) {
+        int slot;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return order.size() > k || deviceDone; });
+            if (order.size() <= k) break;
+            slot = order[k];
+        }
+        processSlot(*slots[size_t(slot)]);
+        std::lock_guard<std::mutex> lk(mu);
+        filled[slot] = 0;
+        cv.notify_all();
     }
+    device.join();
+    if (!deviceError.empty()) throw std::runtime_error("engine: " + deviceError);
+    ResultStore anchors = mergeStores(anchorParts), cands = mergeStores(candParts);
     const double t2 = now_s();
     std::printf("# elapsed time (STEP2): %.6fs\n", t2 - t1);
     std::printf("Correction throughput : ~%.1f reads/second.\n", double(numAnchors) / (t2 - t1));

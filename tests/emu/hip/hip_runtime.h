@@ -608,4 +608,10 @@ inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind
     return hipSuccess;
 }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+// a device with 8 GiB free
+inline hipError_t hipMemGetInfo(size_t* freeB, size_t* totalB) {
+    *freeB = size_t(8) << 30;
+    *totalB = size_t(16) << 30;
+    return hipSuccess;
+}
 inline const char* hipGetErrorString(hipError_t) { return "emulated HIP error"; }

@@ -140,3 +140,19 @@ def test_padded_pitch_upload_matches_minimal_pitch(tmp_path):
     for a, b in zip(runs[0], runs[1]):
         assert np.array_equal(a, b)
     rs.close()
+
+
+def test_candidate_result_overflow_recorrects_in_halves(emu_exe, tmp_path):
+    """With room for one candidate result per anchor, chunks whose candidate
+    corrections do not fit are corrected again in halves: the output is the
+    oracle's."""
+    case = SMALL["se_mixed_small"]
+    want = _oracle(case, str(tmp_path))
+    reads, _ = helpers.simulate(case, str(tmp_path))
+    out = os.path.join(str(tmp_path), "out")
+    p = subprocess.run([emu_exe, "-i", reads, "-o", "r.fastq", "-d", out, "-g", "0", "-t", "4",
+                        "--candidateResultsPerAnchor", "1"] + _args(case, str(tmp_path)),
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert p.returncode == 0, p.stdout[-3000:]
+    assert "corrected in halves" in p.stdout
+    assert helpers.md5(os.path.join(out, "r.fastq")) == want
