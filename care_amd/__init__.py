@@ -48,6 +48,7 @@ class CareOptions(ctypes.Structure):
         ("threshold_anchor", ctypes.c_float),
         ("threshold_cands", ctypes.c_float),
         ("candidate_limit", ctypes.c_int),
+        ("max_refinements", ctypes.c_int),
     ]
 
 

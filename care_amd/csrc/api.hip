@@ -270,6 +270,7 @@ void care_options_default(care_options* o) {
     o->threshold_anchor = 0.5f;
     o->threshold_cands = 0.5f;
     o->candidate_limit = 0;
+    o->max_refinements = 5;
 }
 
 const char* care_last_error(void) { return g_lastError.c_str(); }
@@ -287,6 +288,8 @@ int care_context_create(int device, const care_options* opt, care_context** out)
     if (opt->correction_type_cands == 3 && opt->correction_type != 3)
         return fail(CARE_ERR_ARGUMENT, "correction_type_cands 3 (learned gate) needs correction_type 3");
     if (opt->candidate_limit < 0) return fail(CARE_ERR_ARGUMENT, "candidate_limit must be >= 0");
+    if (opt->max_refinements < 0 || opt->max_refinements > 5)
+        return fail(CARE_ERR_ARGUMENT, "max_refinements must be in 0..5");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CARE_ERR_DEVICE, "no HIP device available");
     if (device < 0 || device >= count) return fail(CARE_ERR_ARGUMENT, "device %d outside 0..%d", device, count - 1);
@@ -664,7 +667,7 @@ int care_minhash_query_async(care_context* ctx, const uint32_t* d_seqs, int seq_
         qa.ids = d_read_ids;
         qa.lens = d_lengths;
         launch_query(ctx->rs, ctx->tables, 0, n, ctx->maxCand, ctx->queryCap, ctx->queryRegMax, o.exclude_ambiguous,
-                     o.correction_type == 3 ? o.candidate_limit : 0, d_candidates, d_counts, spill, nullptr, stream, qa);
+                     o.candidate_limit, d_candidates, d_counts, spill, nullptr, stream, qa);
         HIP_TRY(hipGetLastError());
     } catch (const HipError& e) {
         return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
@@ -783,7 +786,7 @@ int care_minhash_query(care_context* ctx, uint64_t first, int n, uint32_t* ids, 
         for (int done = 0; done < n; done += B) {
             const int nb = std::min(B, n - done);
             launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->queryCap, ctx->queryRegMax,
-                         ctx->opt.exclude_ambiguous, ctx->opt.correction_type == 3 ? ctx->opt.candidate_limit : 0,
+                         ctx->opt.exclude_ambiguous, ctx->opt.candidate_limit,
                          ctx->candIds.p, ctx->candCount.p, reinterpret_cast<u32*>(ctx->metaA.p), nullptr,
                          ctx->stream);
             HIP_TRY(hipGetLastError());
@@ -872,6 +875,7 @@ int prepare_params(care_context* ctx, CorrectParams& P) {
     P.correctionType = o.correction_type;
     P.correctionTypeCands = o.correction_type_cands;
     P.estimatedCoverage = o.coverage;
+    P.maxRefinements = o.max_refinements;
     if (o.correction_type == 1) {
         if (!ctx->forest[0].loaded) return fail(CARE_ERR_STATE, "correction_type 1 needs an anchor forest (care_forest_load)");
         P.forestA = ctx->forest[0].dev;
@@ -996,7 +1000,7 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
             HIP_TRY(hipEventRecord(e[0], s));
             launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->queryCap, ctx->queryRegMax,
                          o.exclude_ambiguous,
-                         o.correction_type == 3 ? o.candidate_limit : 0, ctx->candIds.p, ctx->candCount.p,
+                         o.candidate_limit, ctx->candIds.p, ctx->candCount.p,
                          reinterpret_cast<u32*>(ctx->metaA.p), ctx->rawCount.p, s);
             launch_sum_counts(ctx->rawCount.p, ctx->candCount.p, nb, ctx->rawHits.p, s);
             HIP_TRY(hipEventRecord(e[1], s));
@@ -1548,7 +1552,7 @@ int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen)
         if (n == 0) n = 2 > int(ctx->rs.n - first) ? 1 : 2;
         ensure_batch(ctx, B);
         hipStream_t s = ctx->stream;
-        const int candLimit = o.correction_type == 3 ? o.candidate_limit : 0;
+        const int candLimit = o.candidate_limit;
         auto timed = [&](auto&& launch) {
             float best = 1e30f;
             for (int rep = 0; rep < 3; rep++) {

@@ -913,7 +913,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         // new_columns past it, gate features two columns past a corrected read
         const int margin = P.correctCandidates ? P.newColumns + 2 : 2;
         Msa msa = build_msa(P, L, dims, cur, m, la, margin, &sMin, &sMax);
-        const int maxIter = kGated ? 0 : 5;  // the gated mode skips refinement
+        const int maxIter = kGated ? 0 : P.maxRefinements;  // the gated mode skips refinement
         for (int iter = 0; iter < maxIter && m > 0; iter++) {
             // src/msa.cpp:532-734
             if (tid == 0) {

@@ -133,6 +133,7 @@ struct CorrectParams {
     DevForest forestA;
     DevForest forestC;
     float estimatedCoverage;
+    int maxRefinements;         // different-region refinement rounds (the reference: 5)
     // learned gate (correctionType 3): BF16 weight matrices and the FP32
     // parameter vector of the gate, logit threshold of acceptance
     const unsigned short* gateW1;

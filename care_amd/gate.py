@@ -9,6 +9,8 @@ the device (care_mlp_train: BF16 MFMA forward/backward, Adam).
 
     python -m care_amd.gate train --seeds 8 --out care_amd/gate_repeat_pe.bin
     python -m care_amd.gate evaluate --gate care_amd/gate_repeat_pe.bin
+    python -m care_amd.gate train-forest --out care_amd/forest_repeat_pe.rf
+    python -m care_amd.gate ablation
 """
 import argparse
 import re
@@ -249,6 +251,156 @@ def evaluate(args):
     return res
 
 
+# ---------------------------------------------------------------------------
+# The method without the gate, on the same reduced pipeline (the top
+# CANDIDATE_LIMIT candidates, no refinement): the classic threshold decision
+# and a random forest in the reference's .rf format over the reference's
+# anchor features (extract_anchor_v3, include/classification.hpp:748-800),
+# trained like the reference trains its forests (sklearn's
+# RandomForestClassifier, ml/care.py:231-282) on the same decision columns
+# as the gate.
+# ---------------------------------------------------------------------------
+DEFAULT_FOREST = os.path.join(HERE, "forest_repeat_pe.rf")
+REDUCED_ARGS = ["--candidateLimit", str(CANDIDATE_LIMIT), "--refinements", "0"]
+# the 13 anchor features of extract_anchor_v3 are 13 of the gate's 32
+# (same operands, same float divisions): orig count / coverage, orig weight
+# / weight sum, orig weight / orig count, consensus count / coverage,
+# support, consensus weight / consensus count, weight sum / coverage,
+# coverage / estimated coverage, weight sum / estimated coverage, average
+# and minimum support, maximum and minimum coverage / estimated coverage
+V3_FROM_GATE = [0, 1, 4, 2, 3, 5, 30, 6, 7, 12, 13, 15, 14]
+
+
+def write_forest(clf, path, desc="13 extract_anchor_v3"):
+    """A fitted sklearn forest in CARE's .rf format (include/forest.hpp:38-55):
+    u64 descriptor length and descriptor, u32 tree count, per tree the u32
+    internal-node count and the nodes in preorder (feature u8, split f64,
+    flag u8: bit 1 left child is a leaf, bit 0 right child is a leaf), a
+    leaf being the f32 fraction of positive samples."""
+    import struct
+    out = bytearray()
+    d = desc.encode()
+    out += struct.pack("<Q", len(d)) + d + struct.pack("<I", len(clf.estimators_))
+    for est in clf.estimators_:
+        t = est.tree_
+        left, right = t.children_left, t.children_right
+
+        def leaf(i):
+            return left[i] == right[i]
+
+        body = bytearray()
+        stack = [0]
+        while stack:
+            i = stack.pop()
+            if leaf(i):
+                v = t.value[i][0]
+                body += struct.pack("<f", float(v[1] / (v[0] + v[1])))
+                continue
+            flag = (2 if leaf(left[i]) else 0) | (1 if leaf(right[i]) else 0)
+            body += struct.pack("<B", int(t.feature[i])) + struct.pack("<d", float(t.threshold[i])) + struct.pack(
+                "<B", flag)
+            stack.append(right[i])
+            stack.append(left[i])
+        internal = int(np.sum(left != right))
+        out += struct.pack("<I", internal) + body
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+
+
+def train_forest(args):
+    from sklearn.ensemble import RandomForestClassifier
+    work = args.work or tempfile.mkdtemp(prefix="care_forest_")
+    os.makedirs(work, exist_ok=True)
+    seeds = [TRAIN_SEED_BASE + 1000 + i for i in range(args.seeds)]
+    xs, ys = [], []
+    for reads, truth in simulate_many(seeds, work, workers=min(8, len(seeds))):
+        x, y = collect(reads, truth)
+        xs.append(x[:, V3_FROM_GATE])
+        ys.append(y)
+        os.remove(reads)
+        os.remove(truth)
+    x = np.concatenate(xs)
+    y = np.concatenate(ys)
+    rng = np.random.default_rng(args.seed)
+    if len(y) > args.rows:
+        pick = rng.choice(len(y), size=args.rows, replace=False)
+        x, y = x[pick], y[pick]
+    t0 = time.perf_counter()
+    clf = RandomForestClassifier(n_estimators=args.trees, max_leaf_nodes=args.leaves, n_jobs=args.jobs,
+                                 random_state=args.seed).fit(x.astype(np.float32), y)
+    info = {"columns": int(len(y)), "positive": float(y.mean()), "trees": args.trees, "max_leaf_nodes": args.leaves,
+            "fit_seconds": time.perf_counter() - t0, "train_accuracy": float(clf.score(x, y))}
+    print("forest: " + json.dumps(info), flush=True)
+    write_forest(clf, args.out)
+    with open(args.out + ".json", "w") as f:
+        json.dump(info, f, indent=1)
+    select_forest(args.out, work)
+
+
+def _forest_args(path, threshold):
+    return ["--correctionType", "1", "--ml-forestfile", os.path.abspath(path), "--thresholdAnchor",
+            repr(threshold)] + REDUCED_ARGS
+
+
+FOREST_SWEEP = [0.5, 0.6, 0.7, 0.8, 0.9, 0.95]
+
+
+def select_forest(path, work):
+    """Forest acceptance threshold on the validation read set, by the gate's
+    rule (fewest FP with TP >= 0.995 plain TP and FP <= plain FP; else the
+    threshold with the most TP among those with FP <= plain FP)."""
+    (vreads, vtruth), = simulate_many([VALIDATION_SEED], work, workers=1)
+    plain = care_amd.evaluate(vreads, _run_cli(vreads, os.path.join(work, "vplain"), []), vtruth)
+    sweep, best = {}, None
+    for t in FOREST_SWEEP:
+        r = care_amd.evaluate(vreads, _run_cli(vreads, os.path.join(work, "f%g" % t), _forest_args(path, t)), vtruth)
+        sweep[repr(t)] = {"TP": r["TP"], "FP": r["FP"], "FN": r["FN"]}
+        print("forest: validation threshold %g: TP %d FP %d (plain TP %d FP %d)" % (t, r["TP"], r["FP"], plain["TP"],
+                                                                                  plain["FP"]), flush=True)
+    ok = [t for t in FOREST_SWEEP if sweep[repr(t)]["TP"] >= 0.995 * plain["TP"] and sweep[repr(t)]["FP"] <= plain["FP"]]
+    if ok:
+        best = min(ok, key=lambda t: sweep[repr(t)]["FP"])
+    else:
+        le = [t for t in FOREST_SWEEP if sweep[repr(t)]["FP"] <= plain["FP"]] or FOREST_SWEEP
+        best = max(le, key=lambda t: sweep[repr(t)]["TP"])
+    try:
+        with open(path + ".json") as f:
+            info = json.load(f)
+    except (OSError, ValueError):
+        info = {}
+    info["validation"] = {"seed": VALIDATION_SEED, "plain": {k: plain[k] for k in ("TP", "FP", "FN")}, "sweep": sweep}
+    info["threshold"] = best
+    print("forest: chosen threshold %g" % best, flush=True)
+    with open(path + ".json", "w") as f:
+        json.dump(info, f, indent=1)
+
+
+def ablation(reads, truth, work, gate_path=DEFAULT_GATE, forest_path=DEFAULT_FOREST):
+    """What the gate gains over the method without it: the full plain path,
+    and the reduced pipeline (top CANDIDATE_LIMIT candidates by map hits, no
+    refinement) deciding its non-consensus anchor columns by the classic
+    thresholds, by the forest, by the gate, and by accepting every
+    consensus.  Base-level TP/FP/FN and STEP2 seconds of the command line."""
+    rows = {"plain": [], "reduced_classic": REDUCED_ARGS}
+    if os.path.exists(forest_path):
+        rows["reduced_forest"] = _forest_args(forest_path, chosen_threshold(forest_path))
+    if os.path.exists(gate_path):
+        rows["reduced_mlp_gate"] = _gated_args(gate_path, chosen_threshold(gate_path))
+        # a gate that accepts every column: logit threshold far below any output
+        rows["reduced_accept_all"] = _gated_args(gate_path, 1e-6)
+    res = {}
+    for name, extra in rows.items():
+        outdir = os.path.join(work, "ablation_" + name)
+        cmd = [care_amd.CARE_GPU, "-i", reads, "-o", "c.fastq", "-d", outdir, "-g", "0", "-t", "16"] + CLI_ARGS + extra
+        p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if p.returncode != 0:
+            raise RuntimeError(p.stdout[-2000:])
+        r = care_amd.evaluate(reads, os.path.join(outdir, "c.fastq"), truth)
+        m = re.search(r"elapsed time \(STEP2\): ([0-9.]+)s", p.stdout)
+        res[name] = {"TP": r["TP"], "FP": r["FP"], "FN": r["FN"], "step2_s": float(m.group(1)) if m else None}
+    return res
+
+
 LARGE_ARGS = ["-c", "40", "--pairmode", "PE", "-q", "-k", "20", "-h", "48", "--candidateCorrection"]
 
 
@@ -339,9 +491,28 @@ def main(argv=None):
     el.add_argument("--genome", type=int, default=10_000_000)
     el.add_argument("--work", default=None)
     el.add_argument("--thresholds-cands", type=lambda v: [float(x) for x in v.split(",")], default=[0.5, 0.9, 0.99])
+    tf = sub.add_parser("train-forest")
+    tf.add_argument("--seeds", type=int, default=4)
+    tf.add_argument("--rows", type=int, default=2_000_000)
+    tf.add_argument("--trees", type=int, default=64)
+    tf.add_argument("--leaves", type=int, default=512)
+    tf.add_argument("--jobs", type=int, default=16)
+    tf.add_argument("--seed", type=int, default=1)
+    tf.add_argument("--work", default=None)
+    tf.add_argument("--out", default=DEFAULT_FOREST)
+    ab = sub.add_parser("ablation")
+    ab.add_argument("--seed", type=int, default=EVAL_SEED)
+    ab.add_argument("--work", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "train":
         train(a)
+    elif a.cmd == "train-forest":
+        train_forest(a)
+    elif a.cmd == "ablation":
+        work = a.work or tempfile.mkdtemp(prefix="care_ablation_")
+        os.makedirs(work, exist_ok=True)
+        (reads, truth), = simulate_many([a.seed], work, workers=1)
+        print(json.dumps(ablation(reads, truth, work)), flush=True)
     elif a.cmd == "timing":
         timing(a.rows, a.epochs)
     elif a.cmd == "evaluate-large":

@@ -386,6 +386,7 @@ HostOptions parseOptions(int argc, char** argv) {
         else if (a == "--ml-gatefile") o.gateFile = value();
         else if (a == "--ml-cands-gatefile") o.gateFileCands = value();
         else if (a == "--candidateLimit") o.eng.candidate_limit = std::stoi(value());
+        else if (a == "--refinements") o.eng.max_refinements = std::stoi(value());
         else if (a == "--ml-cands-forestfile") o.forestCands = value();
         else if (a == "--thresholdAnchor") o.eng.threshold_anchor = std::stof(value());
         else if (a == "--thresholdCands") o.eng.threshold_cands = std::stof(value());

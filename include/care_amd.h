@@ -50,8 +50,12 @@ typedef struct care_options {
                                     (with correction_type 3) */
     float threshold_anchor;      /* --thresholdAnchor */
     float threshold_cands;       /* --thresholdCands */
-    int candidate_limit;         /* correction_type 3 only: keep the candidates hit by the
-                                    most maps, at most this many per anchor (0 = all) */
+    int candidate_limit;         /* keep the candidates hit by the most maps, at most this many
+                                    per anchor, ties to the smaller read id (0 = all, the
+                                    reference's behaviour) */
+    int max_refinements;         /* rounds of different-region refinement, 0..5 (5 = the
+                                    reference, include/corrector.hpp:1309; correction_type 3
+                                    always skips refinement) */
 } care_options;
 
 typedef struct care_context care_context;

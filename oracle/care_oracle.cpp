@@ -64,7 +64,8 @@ struct Options {
     int kmerlength = 20;
     int numHashFunctions = 48;
     int correctionType = 0;      // 0 classic, 1 forest, 3 learned gate (engine extension)
-    int candidateLimit = 0;      // correctionType 3: top candidates by map hits
+    int candidateLimit = 0;      // top candidates by map hits (0 = all, the reference)
+    int maxRefinements = 5;      // different-region refinement rounds (the reference: 5)
     int gateConstant = 1;        // correctionType 3: the oracle's gate accepts (1) or rejects (-1) every column
     std::string gateDump;        // correctionType 3: binary record of every decision column
     int gateConstantCands = 1;   // correctionTypeCands 3: the same for candidate columns
@@ -982,9 +983,10 @@ struct Corrector {
         if (opt.excludeAmbiguousReads && rs.ambig[id]) return ids;
         tables.query(ids, rs.seq(id), rs.lens[id]);
         std::sort(ids.begin(), ids.end());
-        if (opt.correctionType == 3 && opt.candidateLimit > 0) {
-            // the engine's gated mode: keep the candidates hit by the most
-            // maps (ties to the smaller id), after self / ambiguous removal
+        if (opt.candidateLimit > 0) {
+            // the engine's candidate limit (its gated mode and reduced
+            // pipeline): keep the candidates hit by the most maps (ties to
+            // the smaller id), after self / ambiguous removal
             std::vector<std::pair<int, u32>> hc;
             for (std::size_t i = 0; i < ids.size();) {
                 std::size_t j = i;
@@ -1207,7 +1209,7 @@ struct Corrector {
         buildMsa();
 
         // refinement (include/corrector.hpp:1307-1421); none in the gated mode
-        for (int it = 0; it < (opt.correctionType == 3 ? 0 : 5); it++) {
+        for (int it = 0; it < (opt.correctionType == 3 ? 0 : opt.maxRefinements); it++) {
             std::vector<char> remove;
             if (!msa.differentRegion(int(opt.estimatedCoverage), remove)) break;
             std::vector<Cand> kept;
@@ -1559,6 +1561,7 @@ Options parseArgs(int argc, char** argv) {
         else if (a == "--correctionType") o.correctionType = std::stoi(next());
         else if (a == "--correctionTypeCands") o.correctionTypeCands = std::stoi(next());
         else if (a == "--candidateLimit") o.candidateLimit = std::stoi(next());
+        else if (a == "--refinements") o.maxRefinements = std::stoi(next());
         else if (a == "--gateConstant") o.gateConstant = std::stoi(next());
         else if (a == "--gateDump") o.gateDump = next();
         else if (a == "--gateConstantCands") o.gateConstantCands = std::stoi(next());

@@ -27,6 +27,11 @@ SMALL = {
     "se_mixed_small": {"profile": "mixed-se", "seed": 203, "genome": 2000,
                        "args": ["-c", "50", "--pairmode", "SE", "--excludeAmbiguous", "-q", "-k", "20", "-h", "48",
                                 "--candidateCorrection"]},
+    # reduced pipeline (top 20 candidates by map hits, no refinement) with
+    # the classic decision
+    "pe_reduced_small": {"profile": "repeat-pe", "seed": 205, "genome": 2000,
+                         "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "2", "-k", "20", "-h",
+                                  "32", "--candidateLimit", "20", "--refinements", "0"]},
     "pe_forest_small": {"profile": "repeat-pe", "seed": 204, "genome": 1500,
                         "args": ["-c", "30", "--pairmode", "PE", "-q", "-k", "20", "-h", "32",
                                  "--candidateCorrection", "--correctionType", "1", "--correctionTypeCands", "1"],

@@ -111,3 +111,23 @@ def test_end_to_end_matches_reference(name, tmp_path):
     _say(proc.stdout[-2000:])
     assert proc.returncode == 0
     assert helpers.md5(str(out / "gpu.fastq")) == g["output_md5"]
+
+
+def test_reduced_pipeline_matches_oracle(tmp_path):
+    """The reduced pipeline (top 20 candidates by map hits, no refinement)
+    with the classic decision, as in the learned gate's ablation: the
+    device output equals the oracle's."""
+    case = {"profile": "repeat-pe", "seed": 231, "genome": 60000,
+            "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "2", "-k", "20", "-h", "32",
+                     "--candidateLimit", "20", "--refinements", "0"]}
+    _say("reduced pipeline vs oracle")
+    reads, _ = helpers.simulate(case, str(tmp_path))
+    import subprocess
+    gpu = care_amd.run_care_gpu(["-i", reads, "-o", "g.fastq", "-d", str(tmp_path / "g"), "-g", "0", "-t", "8"]
+                                + case["args"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                                timeout=300)
+    assert gpu.returncode == 0, gpu.stdout[-2000:]
+    orc = subprocess.run([helpers.ORACLE_EXE, "-i", reads, "-o", "o.fastq", "-d", str(tmp_path / "o"), "-t", "8"]
+                         + case["args"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert orc.returncode == 0, orc.stdout[-2000:]
+    assert helpers.md5(str(tmp_path / "g" / "g.fastq")) == helpers.md5(str(tmp_path / "o" / "o.fastq"))
