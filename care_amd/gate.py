@@ -342,13 +342,14 @@ def _forest_args(path, threshold):
             repr(threshold)] + REDUCED_ARGS
 
 
-FOREST_SWEEP = [0.5, 0.6, 0.7, 0.8, 0.9, 0.95]
+FOREST_SWEEP = [0.5, 0.7, 0.9, 0.95, 0.97, 0.98, 0.99, 0.995, 0.999]
 
 
 def select_forest(path, work):
     """Forest acceptance threshold on the validation read set, by the gate's
     rule (fewest FP with TP >= 0.995 plain TP and FP <= plain FP; else the
     threshold with the most TP among those with FP <= plain FP)."""
+    os.makedirs(work, exist_ok=True)
     (vreads, vtruth), = simulate_many([VALIDATION_SEED], work, workers=1)
     plain = care_amd.evaluate(vreads, _run_cli(vreads, os.path.join(work, "vplain"), []), vtruth)
     sweep, best = {}, None
@@ -500,6 +501,9 @@ def main(argv=None):
     tf.add_argument("--seed", type=int, default=1)
     tf.add_argument("--work", default=None)
     tf.add_argument("--out", default=DEFAULT_FOREST)
+    sf = sub.add_parser("select-forest")
+    sf.add_argument("--forest", default=DEFAULT_FOREST)
+    sf.add_argument("--work", default=None)
     ab = sub.add_parser("ablation")
     ab.add_argument("--seed", type=int, default=EVAL_SEED)
     ab.add_argument("--work", default=None)
@@ -508,6 +512,8 @@ def main(argv=None):
         train(a)
     elif a.cmd == "train-forest":
         train_forest(a)
+    elif a.cmd == "select-forest":
+        select_forest(a.forest, a.work or tempfile.mkdtemp(prefix="care_forest_sel_"))
     elif a.cmd == "ablation":
         work = a.work or tempfile.mkdtemp(prefix="care_ablation_")
         os.makedirs(work, exist_ok=True)

@@ -623,8 +623,11 @@ int runMain(int argc, char** argv) {
     const double t0 = now_s();
     Library lib = readLibrary(o.inputs, o.eng.paired_end != 0, true);
     if (!lib.fastq) o.eng.use_quality = 0;
+    const double tParsed = now_s();
     std::unique_ptr<care_readset> rs = packLibrary(lib, o.eng.use_quality ? o.qualityBits : 0, threads);
-    std::printf("Number of reads: %llu, max length %d\n", (unsigned long long)rs->n, rs->maxLen);
+    const double tPacked = now_s();
+    std::printf("Number of reads: %llu, max length %d (parsed in %.3fs, packed in %.3fs)\n", (unsigned long long)rs->n,
+                rs->maxLen, tParsed - t0, tPacked - tParsed);
     care_context* ctx = nullptr;
     auto check = [&](int rc) {
         if (rc != CARE_OK) throw std::runtime_error(std::string("engine: ") + care_last_error());
@@ -652,8 +655,12 @@ int runMain(int argc, char** argv) {
         check(care_forest_load(ctx, 1, o.forestCands.c_str(), o.maxTreesCands));
     check(care_readstore_upload(ctx, rs->n, rs->maxLen, rs->seqs.data(), rs->pitch, rs->lens.data(), rs->ambig.data(),
                                 o.eng.use_quality ? rs->quals.data() : nullptr, rs->qpitch, rs->qbits));
+    const double tUploaded = now_s();
     check(care_minhash_build(ctx));
-    std::printf("# elapsed time (STEP1): %.6fs\n", now_s() - t0);
+    const double tBuilt = now_s();
+    std::printf("Device: context and read store upload %.3fs, minhash tables %.3fs\n", tUploaded - tPacked,
+                tBuilt - tUploaded);
+    std::printf("# elapsed time (STEP1): %.6fs\n", tBuilt - t0);
 
     // launch shapes timed on the first anchors (set-up, like the table
     // build; results do not depend on them)
