@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1644,3 +1645,166 @@ int care_autotune(care_context* ctx, uint64_t first, int n, care_tuning* chosen)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Correction without allocation: care_correct_batch_noalloc
+// ---------------------------------------------------------------------------
+// The batch scratch (candidate lists, records, per-anchor results, counters,
+// candidate results) of care_correct_batch_noalloc lives in this fixed
+// device array.  Each device holds its own copy from the first load of the
+// library until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress reports for the device.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void* g_scratchAddr[kMaxDevices] = {};
+
+// the device address of `scratch` on the context's device (looked up once
+// per device, with that device current)
+int scratch_address(care_context* ctx, void** out) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return fail(CARE_ERR_DEVICE, "device index %d", ctx->device);
+    std::lock_guard<std::mutex> lk(g_scratchMutex);
+    if (!g_scratchAddr[ctx->device]) {
+        if (hipSetDevice(ctx->device) != hipSuccess) return fail(CARE_ERR_DEVICE, "hipSetDevice failed");
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(CARE_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratchAddr[ctx->device] = p;
+    }
+    *out = g_scratchAddr[ctx->device];
+    return CARE_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// scratch bytes of a device batch of b anchors
+size_t noalloc_bytes(const care_context* ctx, int b) {
+    const size_t mc = size_t(ctx->maxCand);
+    const size_t outPitch = size_t(ctx->rs.maxLen + 15) / 16 * 16;
+    const size_t candCap = ctx->opt.correct_candidates ? size_t(b) * 64 : 0;
+    return align256(sizeof(int) * 4) + align256(sizeof(u32) * size_t(b) * mc) + align256(sizeof(int) * size_t(b)) +
+           2 * align256(sizeof(CandMeta) * size_t(b) * mc) + align256(size_t(b)) + align256(size_t(b) * outPitch) +
+           align256(sizeof(u32) * std::max<size_t>(candCap, 1)) + align256(std::max<size_t>(candCap, 1) * outPitch);
+}
+
+}  // namespace
+
+extern "C" int care_correct_batch_noalloc(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
+    if (!ctx || !r || !r->anchor_status || !r->anchor_sequences) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
+    if (n < 0 || first + uint64_t(n) > ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "range outside read store");
+    if (r->sequence_pitch < ctx->rs.maxLen) return fail(CARE_ERR_ARGUMENT, "sequence_pitch < max read length");
+    if (ctx->opt.paired_end && ((first & 1) || (n & 1)))
+        return fail(CARE_ERR_ARGUMENT, "paired-end batches must start at an even id and hold whole pairs");
+    if (ctx->opt.correct_candidates && (!r->candidate_read_ids || !r->candidate_sequences))
+        return fail(CARE_ERR_ARGUMENT, "candidate output buffers required with correct_candidates");
+    if (ctx->opt.correction_type == 3 && ctx->record.on)
+        return fail(CARE_ERR_STATE, "gate column recording is not available without allocation");
+    r->num_candidates = 0;
+    if (n == 0) return CARE_OK;
+    // the largest device batch whose scratch fits the fixed array
+    const int step = ctx->opt.paired_end ? 2 : 1;
+    int B = 0;
+    for (int lo = 0, hi = std::min(n + (n & 1), 1 << 20); lo <= hi;) {
+        const int mid = (lo + hi) / 2;
+        if (noalloc_bytes(ctx, std::max(mid, 1)) <= kNoAllocScratchBytes) {
+            B = mid;
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    B -= B % step;
+    if (B < step)
+        return fail(CARE_ERR_CAPACITY, "%s anchor needs %zu bytes of scratch, more than the fixed %zu",
+                    step == 2 ? "a pair of" : "one", noalloc_bytes(ctx, step), kNoAllocScratchBytes);
+    const auto& o = ctx->opt;
+    CorrectParams P{};
+    if (const int rc = prepare_params(ctx, P)) return rc;
+    void* base = nullptr;
+    if (const int rc = scratch_address(ctx, &base)) return rc;
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        const int T = ctx->blockThreads;
+        const int gatedK = correct_gate_mode(o.correction_type, o.correct_candidates, o.correction_type_cands);
+        const size_t lds = correct_lds_bytes(ctx->rs.maxLen, o.use_quality, ctx->rs.qpitch, T, gatedK, ctx->msaChunk);
+        if (lds > size_t(kMaxLdsBytes))
+            return fail(CARE_ERR_CAPACITY, "reads of length %d need %zu bytes of LDS per correction workgroup (limit %d)",
+                        ctx->rs.maxLen, lds, kMaxLdsBytes);
+        const int blocksPerCU = ctx->blocksPerCU > 0 ? ctx->blocksPerCU : correct_resident_blocks(T, lds, gatedK);
+        // carve the scratch for batches of B anchors
+        const size_t mc = size_t(ctx->maxCand);
+        const int outPitch = (ctx->rs.maxLen + 15) / 16 * 16;
+        const int candCapB = o.correct_candidates ? B * 64 : 0;
+        char* p = static_cast<char*>(base);
+        auto take = [&](size_t bytes) {
+            char* q = p;
+            p += align256(bytes);
+            return q;
+        };
+        int* counters = reinterpret_cast<int*>(take(sizeof(int) * 4));
+        u32* candIds = reinterpret_cast<u32*>(take(sizeof(u32) * size_t(B) * mc));
+        int* candCount = reinterpret_cast<int*>(take(sizeof(int) * size_t(B)));
+        CandMeta* metaA = reinterpret_cast<CandMeta*>(take(sizeof(CandMeta) * size_t(B) * mc));
+        CandMeta* metaB = reinterpret_cast<CandMeta*>(take(sizeof(CandMeta) * size_t(B) * mc));
+        u8* outStatus = reinterpret_cast<u8*>(take(size_t(B)));
+        char* outSeq = take(size_t(B) * size_t(outPitch));
+        u32* candOutIds = reinterpret_cast<u32*>(take(sizeof(u32) * std::max<size_t>(size_t(candCapB), 1)));
+        char* candOutSeq = take(std::max<size_t>(size_t(candCapB), 1) * size_t(outPitch));
+        hipStream_t s = ctx->stream;
+        P.candIds = candIds;
+        P.candCount = candCount;
+        P.metaA = metaA;
+        P.metaB = metaB;
+        P.outPitch = outPitch;
+        P.candOutIds = candOutIds;
+        P.candOutSeq = candOutSeq;
+        P.candOutCount = counters + 1;
+        P.queueHead = counters;
+        P.gateCount = nullptr;
+        int done_cands = 0;
+        for (int done = 0; done < n; done += B) {
+            const int nb = std::min(B, n - done);
+            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->queryCap, ctx->queryRegMax,
+                         o.exclude_ambiguous, o.candidate_limit, candIds, candCount, reinterpret_cast<u32*>(metaA),
+                         nullptr, s);
+            // queue head and candidate result count start at zero in every batch
+            launch_reset_counters(counters, counters + 1, s);
+            P.firstAnchor = first + done;
+            P.numAnchors = nb;
+            P.outStatus = outStatus;
+            P.outSeq = outSeq;
+            P.candOutCap = o.correct_candidates ? std::min(candCapB, std::max(0, r->candidate_capacity - done_cands)) : 0;
+            const int blocks = std::max(1, std::min(nb, ctx->numCUs * blocksPerCU));
+            launch_correct(P, blocks, T, ctx->msaChunk, s);
+            HIP_TRY(hipGetLastError());
+            int nc = 0;
+            if (o.correct_candidates) HIP_TRY(hipMemcpyAsync(&nc, counters + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(r->anchor_status + done, outStatus, size_t(nb), hipMemcpyDeviceToHost, s));
+            copy_rows_to_host(r->anchor_sequences + size_t(done) * size_t(r->sequence_pitch), size_t(r->sequence_pitch),
+                              outSeq, size_t(outPitch), size_t(ctx->rs.maxLen), size_t(nb), s);
+            HIP_TRY(hipStreamSynchronize(s));
+            if (o.correct_candidates) {
+                if (nc > P.candOutCap) return fail(CARE_ERR_CAPACITY, "candidate_capacity exceeded (%d results)", done_cands + nc);
+                if (nc > 0) {
+                    HIP_TRY(hipMemcpyAsync(r->candidate_read_ids + done_cands, candOutIds, sizeof(u32) * size_t(nc),
+                                           hipMemcpyDeviceToHost, s));
+                    copy_rows_to_host(r->candidate_sequences + size_t(done_cands) * size_t(r->sequence_pitch),
+                                      size_t(r->sequence_pitch), candOutSeq, size_t(outPitch), size_t(ctx->rs.maxLen),
+                                      size_t(nc), s);
+                    HIP_TRY(hipStreamSynchronize(s));
+                }
+                done_cands += nc;
+                r->num_candidates = done_cands;
+            }
+        }
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    return CARE_OK;
+}

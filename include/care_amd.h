@@ -193,6 +193,19 @@ typedef struct care_batch_result {
  * candidatecorrection,correction}kernels.cu). */
 int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* result);
 
+/* care_correct_batch without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as care_correct_batch.  Its batch scratch is a
+ * fixed 256 MiB device array of the library (one per device, held until the
+ * process ends): anchors go through in device batches that fit it (about
+ * 1,460 anchors at -h 48 -c 40, 3,000 at -h 32 -c 30), each batch's results
+ * copied to the result arrays before the next starts; CARE_ERR_CAPACITY if
+ * one anchor (one pair) needs more scratch than the array holds.  Calls
+ * that overlap on one device (other threads or contexts) race on the array.
+ * Gate column recording (care_gate_columns) is not available here. */
+int care_correct_batch_noalloc(care_context* ctx, uint64_t first, int n, care_batch_result* result);
+
 /* Number of anchors per internal launch (even, 0 = automatic) and threads
  * per correction workgroup (128, 192 or 256, 0 = keep); the other launch shapes
  * stay as set by care_set_tuning / care_autotune.  The reference fixes these

@@ -35,6 +35,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(x)
 #define __shared__ static
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (reinterpret_cast<const void*>(&(x)))
 #define HIP_DYNAMIC_SHARED(type, var) type* var = reinterpret_cast<type*>(emu::dyn_lds());
 
 struct dim3 {
@@ -608,6 +610,11 @@ inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind
     return hipSuccess;
 }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+// the address of a __device__ variable (host memory in the emulation)
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+    *p = const_cast<void*>(symbol);
+    return hipSuccess;
+}
 // a device with 8 GiB free
 inline hipError_t hipMemGetInfo(size_t* freeB, size_t* totalB) {
     *freeB = size_t(8) << 30;
