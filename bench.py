@@ -15,7 +15,9 @@ MLP-gated path (top candidates by map hits, no refinement, a learned gate
 on every non-HQ decision column; weights care_amd/gate_repeat_pe.bin,
 trained on other seeds of the same generator).  `value` is the gated
 path's reads/s; both paths' reads/s and their base-level TP/FP/FN on the
-read set's truth (care-gpu command line output) are in `config`.
+read set's truth (care-gpu command line output) are in `config`, with the
+ablation of the gate: the same reduced pipeline deciding by the classic
+thresholds, by a random forest and by accepting every consensus.
 
 Prints one JSON line (see the fields below).  --gpus above 1 is refused: this
 build runs on one device.
@@ -316,6 +318,15 @@ def main():
         if gated_ok:
             qual["gated"] = quality(care_amd, reads, truth, w, gate_mod._gated_args(gate_path, threshold),
                                     os.path.join(work, "gated"))
+            # the method without the gate on the same reduced pipeline: the
+            # classic decision, the forest, and accepting every consensus
+            for name, extra in (("reduced_classic", gate_mod.REDUCED_ARGS),
+                                ("reduced_forest", gate_mod._forest_args(gate_mod.DEFAULT_FOREST,
+                                                                         gate_mod.chosen_threshold(gate_mod.DEFAULT_FOREST))),
+                                ("reduced_accept_all", gate_mod._gated_args(gate_path, 1e-6))):
+                if name == "reduced_forest" and not os.path.exists(gate_mod.DEFAULT_FOREST):
+                    continue
+                qual[name] = quality(care_amd, reads, truth, w, extra, os.path.join(work, name))
         log("bench: quality %s" % json.dumps(qual))
 
     # the gated path is the headline only where it meets the target's
