@@ -72,6 +72,24 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_correct_k_query_repeat-pe.txt")
 
 
+SQ_FILE = os.path.join(ROOT, "profiles", "sq_counters_k_correct_k_query_repeat-pe.txt")
+# VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, 32-bit lane operations per second
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def measured_valu_per_anchor(kernel, workload):
+    """VALU instructions per anchor of `kernel` from the committed counter summary, or None."""
+    try:
+        with open(SQ_FILE) as f:
+            for line in f:
+                part = line.split()
+                if len(part) == 4 and part[0] == "valu_per_anchor" and part[1] == kernel and part[2] == workload:
+                    return float(part[3])
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic(kernel, workload):
     """Bytes per launch of `kernel` on `workload` from the committed counter summary, or None."""
     try:
@@ -357,6 +375,19 @@ def main():
                 "kernel": kernel, "launches_per_step": r["launches"],
                 "bytes_per_launch": int(alg / r["launches"]), "avg_launch_ms": round(launch_ms, 4)}
 
+    def roofline_valu(r):
+        # issued VALU lane operations per second against the VALU issue peak:
+        # the correction kernels are VALU-issue bound (profiles/sq_counters_*)
+        kernel = "k_correct<%s,false>" % ("true" if r is gated else "false")  # noqa
+        per = measured_valu_per_anchor(kernel, args.workload)
+        if per is None:
+            return None
+        launch_ms = r["correct_ms"] / r["launches"]
+        achieved = per * 64 * (n / r["launches"]) / (launch_ms / 1e3) / 1e12
+        return {"bound": "valu", "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s",
+                "frac": round(achieved / VALU_PEAK_TOPS, 4), "kernel": kernel,
+                "valu_instructions_per_anchor": per, "source": "profiles/sq_counters_k_correct_k_query_repeat-pe.txt"}
+
     def roofline_query(r):
         # per anchor: h stored signatures, h probes of (key, offset/count),
         # the hit values, candidate ids and the count written
@@ -418,8 +449,10 @@ def main():
         "roofline": roofline(head),
         "cpu_baseline": cpu,
     }
+    line["roofline_valu"] = roofline_valu(head)
     if gated is not None:
         line["roofline_plain"] = roofline(plain)
+        line["roofline_valu_plain"] = roofline_valu(plain)
     line["roofline_query"] = roofline_query(plain)
     print(json.dumps(line), flush=True)
 
