@@ -91,6 +91,18 @@ class CareTuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MapView(ctypes.Structure):
+    """One minhash map in key-grouped form (care_map_view)."""
+    _fields_ = [
+        ("num_keys", ctypes.c_uint64),
+        ("keys", ctypes.c_void_p),
+        ("offsets", ctypes.c_void_p),
+        ("counts", ctypes.c_void_p),
+        ("num_values", ctypes.c_uint64),
+        ("values", ctypes.c_void_p),
+    ]
+
+
 MLP_FEATURES = 32
 MLP_PARAMS = 4737
 
@@ -119,6 +131,14 @@ ENGINE_SYMBOLS = {
     "care_readstore_gather": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                              ctypes.c_int, ctypes.c_void_p]),
     "care_minhash_build": (ctypes.c_int, [ctypes.c_void_p]),
+    "care_minhash_import": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MapView), ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int]),
+    "care_minhash_geometry": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int)]),
+    "care_minhash_map_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.POINTER(ctypes.c_uint64)]),
+    "care_minhash_export_map": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p]),
     "care_minhash_signatures": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]),
     "care_minhash_query": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_uint64]),
@@ -185,6 +205,11 @@ HOST_SYMBOLS = {
     "care_readset_lengths": (ctypes.c_void_p, [ctypes.c_void_p]),
     "care_readset_ambiguous": (ctypes.c_void_p, [ctypes.c_void_p]),
     "care_readset_quals": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "care_readset_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    "care_readset_load_preprocessed": (ctypes.c_void_p, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    "care_minhash_save_file": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_float]),
+    "care_minhash_load_file": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int)]),
 }
 
 _engine = None
@@ -259,10 +284,13 @@ def default_options(**kw):
 class ReadSet:
     """Packed reads (2-bit, N rule, binned qualities) parsed by the C++ host."""
 
-    def __init__(self, files, paired_end=False, qual_bits=0):
+    def __init__(self, files, paired_end=False, qual_bits=0, _handle=None):
         h = host()
-        arr = (ctypes.c_char_p * len(files))(*[f.encode() for f in files])
-        self._p = h.care_readset_load(arr, len(files), int(paired_end), int(qual_bits))
+        if _handle is None:
+            arr = (ctypes.c_char_p * len(files))(*[f.encode() for f in files])
+            self._p = h.care_readset_load(arr, len(files), int(paired_end), int(qual_bits))
+        else:
+            self._p = _handle
         if not self._p:
             raise CareError("read loading failed: %s" % h.care_host_last_error().decode())
         self.n = int(h.care_readset_size(self._p))
@@ -293,6 +321,19 @@ class ReadSet:
     @property
     def quals(self):
         return self._array(host().care_readset_quals, np.uint32, self.n * self.qpitch).reshape(self.n, max(self.qpitch, 1))
+
+    def save(self, path, qual_bits=8):
+        """Writes the reference's preprocessed-reads file (care_readset_save)."""
+        if host().care_readset_save(self._p, path.encode(), qual_bits) != 0:
+            raise CareError(host().care_host_last_error().decode())
+
+    @classmethod
+    def load_preprocessed(cls, path, use_quality=True, qual_bits=8):
+        """Reads a preprocessed-reads file (care_readset_load_preprocessed)."""
+        p = host().care_readset_load_preprocessed(path.encode(), int(use_quality), int(qual_bits))
+        if not p:
+            raise CareError(host().care_host_last_error().decode())
+        return cls(None, qual_bits=qual_bits if use_quality else 0, _handle=p)
 
     def close(self):
         if self._p:
@@ -355,6 +396,34 @@ class Engine:
 
     def build(self):
         self._ck(self.lib.care_minhash_build(self.ctx))
+
+    def export_map(self, m):
+        """Map m in key-grouped form: (keys ascending, offsets, counts, values)."""
+        nk, nv = ctypes.c_uint64(), ctypes.c_uint64()
+        self._ck(self.lib.care_minhash_map_size(self.ctx, m, ctypes.byref(nk), ctypes.byref(nv)))
+        keys = np.zeros(nk.value, dtype=np.uint64)
+        offs = np.zeros(nk.value, dtype=np.uint32)
+        cnts = np.zeros(nk.value, dtype=np.uint32)
+        vals = np.zeros(nv.value, dtype=np.uint32)
+        self._ck(self.lib.care_minhash_export_map(self.ctx, m, keys.ctypes.data, offs.ctypes.data, cnts.ctypes.data,
+                                                  vals.ctypes.data))
+        return keys, offs, cnts, vals
+
+    def import_maps(self, maps, kmer_length, results_per_map):
+        """Installs prebuilt maps [(keys, offsets, counts, values), ...] in
+        place of build()."""
+        arrs = []
+        views = (MapView * len(maps))()
+        for i, (k, o, c, v) in enumerate(maps):
+            k = np.ascontiguousarray(k, dtype=np.uint64)
+            o = np.ascontiguousarray(o, dtype=np.uint32)
+            c = np.ascontiguousarray(c, dtype=np.uint32)
+            v = np.ascontiguousarray(v, dtype=np.uint32)
+            arrs.append((k, o, c, v))
+            views[i] = MapView(len(k), k.ctypes.data, o.ctypes.data, c.ctypes.data, len(v), v.ctypes.data)
+        self._ck(self.lib.care_minhash_import(self.ctx, views, len(maps), kmer_length, results_per_map))
+        self.options.num_hash_maps = len(maps)
+        self.options.kmer_length = kmer_length
 
     def set_launch_config(self, batch=0, threads=0):
         self._ck(self.lib.care_set_launch_config(self.ctx, batch, threads))

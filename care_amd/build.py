@@ -55,7 +55,8 @@ def build_host():
     os.makedirs(BUILD, exist_ok=True)
     lib = os.path.join(BUILD, "libcare_host.so")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", lib + ".tmp",
-          os.path.join(HERE, "host", "care_host.cpp"), "-L" + BUILD, "-lcare_amd",
+          os.path.join(HERE, "host", "care_host.cpp"), os.path.join(HERE, "host", "storage_files.cpp"),
+          "-L" + BUILD, "-lcare_amd",
           "-Wl,-rpath,$ORIGIN", "-lz", "-lpthread"])
     os.replace(lib + ".tmp", lib)
     exe = os.path.join(BUILD, "care-gpu")

@@ -31,6 +31,8 @@ void launch_insert_final(const u64* tkeys, const u32* tcount, const u32* toff, u
 void launch_scatter_values(const u64* sigs, u64 n, int h, int map, const u64* tkeys, const u32* toff, u64 tmask,
                            u32* tfill, u32* values, hipStream_t s);
 void launch_sort_buckets(const u32* toff, const u32* tcount, u64 tsize, u32* values, hipStream_t s);
+void launch_insert_triples(const u64* keys, const u32* offs, const u32* counts, u64 numKeys, u64* fkeys,
+                           uint2* foffcnt, u64 fmask, int* overflow, hipStream_t s);
 void launch_reset_counters(int* queue, int* candOut, hipStream_t s);
 }  // namespace care_engine
 
@@ -102,6 +104,8 @@ struct MapStore {
     DevBuf<uint2> offcnt;
     DevBuf<u32> values;
     u64 mask = 0;
+    u64 numKeys = 0;
+    u64 numValues = 0;
 };
 
 int resultsPerMapThreshold(float coverage) {
@@ -746,6 +750,8 @@ int care_minhash_build(care_context* ctx) {
             ms->offcnt.alloc(fsize);
             ms->values.alloc(numValues);
             ms->mask = fsize - 1;
+            ms->numKeys = numKeys;
+            ms->numValues = numValues;
             launch_fill_u64(ms->keys.p, fsize, kEmptyKey, s);
             launch_insert_final(tkeys.p, tcount.p, toff.p, tsize, ms->keys.p, ms->offcnt.p, ms->mask, overflow.p, s);
             launch_scatter_values(sigs.p, n, h, m, tkeys.p, toff.p, tsize - 1, tfill.p, ms->values.p, s);
@@ -768,6 +774,171 @@ int care_minhash_build(care_context* ctx) {
     ctx->tables.kmask = ~u64(0) >> (2 * (32 - ctx->opt.kmer_length));
     ctx->tables.sigs = ctx->sigs.p;
     ctx->hasTables = true;
+    return CARE_OK;
+}
+
+int care_minhash_import(care_context* ctx, const care_map_view* maps, int num_maps, int kmer_length,
+                        int results_per_map) {
+    if (!ctx || !maps) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasReads) return fail(CARE_ERR_STATE, "no read store uploaded");
+    if (num_maps < 1 || num_maps > 64) return fail(CARE_ERR_ARGUMENT, "num_maps %d outside 1..64", num_maps);
+    if (kmer_length < 1 || kmer_length > 31) return fail(CARE_ERR_ARGUMENT, "kmer_length %d outside 1..31", kmer_length);
+    if (results_per_map < 1 || results_per_map > 65535)
+        return fail(CARE_ERR_ARGUMENT, "results_per_map %d outside 1..65535", results_per_map);
+    const u64 n = ctx->rs.n;
+    // every value range inside the map's values, every read id inside the
+    // store, keys distinct: the query kernel relies on all three
+    for (int m = 0; m < num_maps; m++) {
+        const care_map_view& v = maps[m];
+        if ((v.num_keys && (!v.keys || !v.offsets || !v.counts)) || (v.num_values && !v.values))
+            return fail(CARE_ERR_ARGUMENT, "map %d: null array", m);
+        if (v.num_values > 0xFFFFFFFFull) return fail(CARE_ERR_ARGUMENT, "map %d: more than 2^32-1 values", m);
+        for (u64 i = 0; i < v.num_keys; i++) {
+            if (v.counts[i] < 1 || v.counts[i] > u32(results_per_map))
+                return fail(CARE_ERR_ARGUMENT, "map %d: key %llu has %u values, outside 1..%d", m,
+                            (unsigned long long)i, v.counts[i], results_per_map);
+            if (u64(v.offsets[i]) + v.counts[i] > v.num_values)
+                return fail(CARE_ERR_ARGUMENT, "map %d: values of key %llu end beyond num_values", m,
+                            (unsigned long long)i);
+        }
+        for (u64 j = 0; j < v.num_values; j++)
+            if (v.values[j] >= n)
+                return fail(CARE_ERR_ARGUMENT, "map %d: read id %u outside the store of %llu reads", m, v.values[j],
+                            (unsigned long long)n);
+        std::vector<u64> sorted(v.keys, v.keys + v.num_keys);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(CARE_ERR_ARGUMENT, "map %d: repeated key", m);
+        if (!sorted.empty() && sorted.back() == kEmptyKey)
+            return fail(CARE_ERR_ARGUMENT, "map %d: key 2^64-1 is reserved", m);
+    }
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        ctx->hasTables = false;
+        ctx->maps.clear();
+        std::vector<DevMap> hostMaps(num_maps);
+        DevBuf<int> overflow;
+        overflow.alloc(1);
+        for (int m = 0; m < num_maps; m++) {
+            const care_map_view& v = maps[m];
+            auto ms = std::make_unique<MapStore>();
+            u64 fsize = 1;
+            while (fsize < 2 * v.num_keys + 2) fsize <<= 1;
+            ms->keys.alloc(fsize);
+            ms->offcnt.alloc(fsize);
+            ms->values.alloc(v.num_values);
+            ms->mask = fsize - 1;
+            ms->numKeys = v.num_keys;
+            ms->numValues = v.num_values;
+            DevBuf<u64> dkeys;
+            DevBuf<u32> doffs, dcounts;
+            dkeys.alloc(v.num_keys);
+            doffs.alloc(v.num_keys);
+            dcounts.alloc(v.num_keys);
+            if (v.num_keys) {
+                HIP_TRY(hipMemcpy(dkeys.p, v.keys, sizeof(u64) * v.num_keys, hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(doffs.p, v.offsets, sizeof(u32) * v.num_keys, hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(dcounts.p, v.counts, sizeof(u32) * v.num_keys, hipMemcpyHostToDevice));
+            }
+            if (v.num_values)
+                HIP_TRY(hipMemcpy(ms->values.p, v.values, sizeof(u32) * v.num_values, hipMemcpyHostToDevice));
+            launch_fill_u64(ms->keys.p, fsize, kEmptyKey, s);
+            launch_fill_u32(reinterpret_cast<u32*>(overflow.p), 1, 0, s);
+            if (v.num_keys) {
+                launch_insert_triples(dkeys.p, doffs.p, dcounts.p, v.num_keys, ms->keys.p, ms->offcnt.p, ms->mask,
+                                      overflow.p, s);
+                // the query merges each key's values as ascending lists
+                launch_sort_buckets(doffs.p, dcounts.p, v.num_keys, ms->values.p, s);
+            }
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(ctx->hInts.p, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (ctx->hInts.p[0]) return fail(CARE_ERR_DEVICE, "hash table overflow while importing map %d", m);
+            hostMaps[m] = DevMap{ms->keys.p, ms->offcnt.p, ms->values.p, ms->mask};
+            ctx->maps.push_back(std::move(ms));
+        }
+        ctx->devMaps.alloc(num_maps);
+        HIP_TRY(hipMemcpy(ctx->devMaps.p, hostMaps.data(), sizeof(DevMap) * num_maps, hipMemcpyHostToDevice));
+        ctx->opt.kmer_length = kmer_length;
+        ctx->opt.num_hash_maps = num_maps;
+        ctx->cap = results_per_map;
+        ctx->maxCand = num_maps * results_per_map;
+        ctx->queryCap = std::min((ctx->maxCand + 15) / 16 * 16, kDefaultQueryCap);
+        ctx->queryRegMax = 0;
+        ctx->sigs.alloc(n * num_maps);
+        launch_signatures(ctx->rs, 0, int(n), kmer_length, num_maps, ctx->sigs.p, s);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    ctx->tables.maps = ctx->devMaps.p;
+    ctx->tables.h = num_maps;
+    ctx->tables.k = kmer_length;
+    ctx->tables.kmask = ~u64(0) >> (2 * (32 - kmer_length));
+    ctx->tables.sigs = ctx->sigs.p;
+    ctx->hasTables = true;
+    return CARE_OK;
+}
+
+int care_minhash_geometry(care_context* ctx, int* kmer_length, int* num_maps, int* results_per_map) {
+    if (!ctx || !kmer_length || !num_maps || !results_per_map) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
+    *kmer_length = ctx->tables.k;
+    *num_maps = ctx->tables.h;
+    *results_per_map = ctx->cap;
+    return CARE_OK;
+}
+
+int care_minhash_map_size(care_context* ctx, int map, uint64_t* num_keys, uint64_t* num_values) {
+    if (!ctx || !num_keys || !num_values) return fail(CARE_ERR_ARGUMENT, "null argument");
+    if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
+    if (map < 0 || map >= int(ctx->maps.size())) return fail(CARE_ERR_ARGUMENT, "map %d outside 0..%d", map,
+                                                             int(ctx->maps.size()) - 1);
+    *num_keys = ctx->maps[size_t(map)]->numKeys;
+    *num_values = ctx->maps[size_t(map)]->numValues;
+    return CARE_OK;
+}
+
+int care_minhash_export_map(care_context* ctx, int map, uint64_t* keys, uint32_t* offsets, uint32_t* counts,
+                            uint32_t* values) {
+    uint64_t nk = 0, nv = 0;
+    if (const int rc = care_minhash_map_size(ctx, map, &nk, &nv)) return rc;
+    if ((nk && (!keys || !offsets || !counts)) || (nv && !values)) return fail(CARE_ERR_ARGUMENT, "null array");
+    const MapStore& ms = *ctx->maps[size_t(map)];
+    const u64 fsize = ms.mask + 1;
+    std::vector<u64> slotKeys(fsize);
+    std::vector<uint2> slotOffcnt(fsize);
+    std::vector<u32> slotValues(nv);
+    try {
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(slotKeys.data(), ms.keys.p, sizeof(u64) * fsize, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(slotOffcnt.data(), ms.offcnt.p, sizeof(uint2) * fsize, hipMemcpyDeviceToHost));
+        if (nv) HIP_TRY(hipMemcpy(slotValues.data(), ms.values.p, sizeof(u32) * nv, hipMemcpyDeviceToHost));
+    } catch (const HipError& e) {
+        return fail(CARE_ERR_DEVICE, "%s: %s", e.what, hipGetErrorString(e.err));
+    }
+    std::vector<u64> order;
+    order.reserve(nk);
+    for (u64 sl = 0; sl < fsize; sl++)
+        if (slotKeys[sl] != kEmptyKey) order.push_back(sl);
+    if (order.size() != nk) return fail(CARE_ERR_STATE, "map %d holds %zu keys, expected %llu", map, order.size(),
+                                        (unsigned long long)nk);
+    std::sort(order.begin(), order.end(), [&](u64 a, u64 b) { return slotKeys[a] < slotKeys[b]; });
+    u64 next = 0;
+    for (u64 i = 0; i < nk; i++) {
+        const u64 sl = order[i];
+        const uint2 oc = slotOffcnt[sl];
+        if (u64(oc.x) + oc.y > nv || next + oc.y > nv)
+            return fail(CARE_ERR_STATE, "map %d: inconsistent value range", map);
+        keys[i] = slotKeys[sl];
+        offsets[i] = u32(next);
+        counts[i] = oc.y;
+        std::memcpy(values + next, slotValues.data() + oc.x, sizeof(u32) * oc.y);
+        next += oc.y;
+    }
     return CARE_OK;
 }
 

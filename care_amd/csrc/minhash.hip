@@ -268,6 +268,19 @@ __global__ void k_scatter_values(const u64* sigs, u64 n, int h, int map, const u
     }
 }
 
+// (key, offset, count) triples of a prebuilt map into the compact table
+__global__ void k_insert_triples(const u64* keys, const u32* offs, const u32* counts, u64 numKeys, u64* fkeys,
+                                 uint2* foffcnt, u64 fmask, int* overflow) {
+    for (u64 i = blockIdx.x * u64(blockDim.x) + threadIdx.x; i < numKeys; i += u64(gridDim.x) * blockDim.x) {
+        const u64 slot = find_or_insert(fkeys, fmask, keys[i]);
+        if (slot == ~u64(0)) {
+            atomicExch(overflow, 1);
+            continue;
+        }
+        foffcnt[slot] = make_uint2(offs[i], counts[i]);
+    }
+}
+
 // ascending read ids within every bucket (insertion sort, buckets <= cap)
 __global__ void k_sort_buckets(const u32* toff, const u32* tcount, u64 tsize, u32* values) {
     for (u64 s = blockIdx.x * u64(blockDim.x) + threadIdx.x; s < tsize; s += u64(gridDim.x) * blockDim.x) {
@@ -324,6 +337,11 @@ void launch_scatter_values(const u64* sigs, u64 n, int h, int map, const u64* tk
 }
 void launch_sort_buckets(const u32* toff, const u32* tcount, u64 tsize, u32* values, hipStream_t s) {
     hipLaunchKernelGGL(k_sort_buckets, dim3(grid_for(tsize, 256)), dim3(256), 0, s, toff, tcount, tsize, values);
+}
+void launch_insert_triples(const u64* keys, const u32* offs, const u32* counts, u64 numKeys, u64* fkeys,
+                           uint2* foffcnt, u64 fmask, int* overflow, hipStream_t s) {
+    hipLaunchKernelGGL(k_insert_triples, dim3(grid_for(numKeys, 256)), dim3(256), 0, s, keys, offs, counts, numKeys,
+                       fkeys, foffcnt, fmask, overflow);
 }
 
 // ---------------------------------------------------------------------------

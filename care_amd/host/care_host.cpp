@@ -13,6 +13,7 @@
 
 #include "../../include/care_amd.h"
 #include "../../include/care_host.h"
+#include "readset.h"
 
 #include <zlib.h>
 
@@ -33,9 +34,10 @@
 #include <thread>
 #include <vector>
 
-namespace {
+// text of the last host error (care_host_last_error), shared with storage_files.cpp
+__attribute__((visibility("hidden"))) thread_local std::string g_hostError;
 
-thread_local std::string g_hostError;
+namespace {
 
 using u8 = std::uint8_t;
 using u32 = std::uint32_t;
@@ -192,17 +194,6 @@ Library readLibrary(const std::vector<std::string>& files, bool pairedEnd, bool 
 // ---------------------------------------------------------------------------
 // Packed read store (include/chunkedreadstorageconstruction.hpp:67-92,216-300)
 // ---------------------------------------------------------------------------
-struct care_readset {
-    u64 n = 0;
-    int maxLen = 0;
-    int pitch = 0;
-    int qbits = 0;
-    int qpitch = 0;
-    std::vector<u32> seqs;
-    std::vector<int32_t> lens;
-    std::vector<u8> ambig;
-    std::vector<u32> quals;
-};
 
 namespace {
 
@@ -312,6 +303,7 @@ struct HostOptions {
     int batchsize = 0;
     bool autotune = true;  // time launch shapes on a sample of anchors before STEP 2
     int candResultsPerAnchor = 64;  // room for candidate corrections per anchor of a chunk
+    float hashLoadFactor = 0.8f;    // --hashloadfactor: load of the lookups in a written hash-table file
     std::string forestAnchor, forestCands;
     // checkpoint files of the reference (preprocessed reads, hash tables)
     std::string saveReads, loadReads, saveTables, loadTables;
@@ -412,9 +404,14 @@ HostOptions parseOptions(int argc, char** argv) {
         else if (a == "--load-preprocessedreads-from") o.loadReads = value();
         else if (a == "--save-hashtables-to") o.saveTables = value();
         else if (a == "--load-hashtables-from") o.loadTables = value();
+        else if (a == "--hashloadfactor") {
+            o.hashLoadFactor = std::stof(value());
+            if (!(o.hashLoadFactor > 0.f && o.hashLoadFactor <= 1.f))
+                throw std::runtime_error("--hashloadfactor must lie in (0, 1]");
+        }
         // memory limits, the temporary directory and the reference's table
         // implementation choices do not apply to this build's device tables
-        else if (a == "-m" || a == "--memTotal" || a == "--memHashtables" || a == "--tempdir" || a == "--hashloadfactor" ||
+        else if (a == "-m" || a == "--memTotal" || a == "--memHashtables" || a == "--tempdir" ||
                  a == "--warpcore" || a == "--replicateGpuData" || a == "--singlehash" ||
                  a == "--samplingRateAnchor" || a == "--samplingRateCands")
             (void)value();
@@ -604,10 +601,6 @@ int runMain(int argc, char** argv) {
         return 0;
     }
     if (o.gpus.size() > 1) std::fprintf(stderr, "Using only GPU %d of the given list (single-device build)\n", o.gpus[0]);
-    if (!o.saveReads.empty() || !o.loadReads.empty() || !o.saveTables.empty() || !o.loadTables.empty())
-        throw std::runtime_error(
-            "--save-preprocessedreads-to, --load-preprocessedreads-from, --save-hashtables-to and "
-            "--load-hashtables-from are not supported by this build");
     if (o.outputs.size() != 1 && o.outputs.size() != o.inputs.size())
         throw std::runtime_error("number of output files must be 1 or equal to the number of input files");
     if (o.eng.paired_end && o.inputs.size() > 2) throw std::runtime_error("paired-end input takes one or two files");
@@ -624,8 +617,25 @@ int runMain(int argc, char** argv) {
     Library lib = readLibrary(o.inputs, o.eng.paired_end != 0, true);
     if (!lib.fastq) o.eng.use_quality = 0;
     const double tParsed = now_s();
-    std::unique_ptr<care_readset> rs = packLibrary(lib, o.eng.use_quality ? o.qualityBits : 0, threads);
+    std::unique_ptr<care_readset> rs;
+    if (!o.loadReads.empty()) {
+        // the inputs are still read: the output is built from their records
+        rs.reset(care_readset_load_preprocessed(o.loadReads.c_str(), o.eng.use_quality, o.qualityBits));
+        if (!rs) throw std::runtime_error(care_host_last_error());
+        if (rs->n != lib.size())
+            throw std::runtime_error("preprocessed reads file holds " + std::to_string(rs->n) + " reads, the inputs " +
+                                     std::to_string(lib.size()));
+        std::printf("Loaded readstorage from file %s\n", o.loadReads.c_str());
+    } else {
+        rs = packLibrary(lib, o.eng.use_quality ? o.qualityBits : 0, threads);
+    }
     const double tPacked = now_s();
+    if (!o.saveReads.empty()) {
+        std::printf("Saving reads to file %s\n", o.saveReads.c_str());
+        if (care_readset_save(rs.get(), o.saveReads.c_str(), o.qualityBits) != 0)
+            throw std::runtime_error(care_host_last_error());
+        std::printf("Saved reads\n");
+    }
     std::printf("Number of reads: %llu, max length %d (parsed in %.3fs, packed in %.3fs)\n", (unsigned long long)rs->n,
                 rs->maxLen, tParsed - t0, tPacked - tParsed);
     care_context* ctx = nullptr;
@@ -656,7 +666,22 @@ int runMain(int argc, char** argv) {
     check(care_readstore_upload(ctx, rs->n, rs->maxLen, rs->seqs.data(), rs->pitch, rs->lens.data(), rs->ambig.data(),
                                 o.eng.use_quality ? rs->quals.data() : nullptr, rs->qpitch, rs->qbits));
     const double tUploaded = now_s();
-    check(care_minhash_build(ctx));
+    if (!o.loadTables.empty()) {
+        int loaded = 0;
+        if (care_minhash_load_file(ctx, o.loadTables.c_str(), o.eng.num_hash_maps, &loaded) != 0)
+            throw std::runtime_error(care_host_last_error());
+        int ratio = 0;
+        check(care_minhash_geometry(ctx, &o.eng.kmer_length, &o.eng.num_hash_maps, &ratio));
+        std::printf("Loaded %d hash tables from %s\n", loaded, o.loadTables.c_str());
+    } else {
+        check(care_minhash_build(ctx));
+    }
+    if (!o.saveTables.empty()) {
+        std::printf("Saving minhasher to file %s\n", o.saveTables.c_str());
+        if (care_minhash_save_file(ctx, o.saveTables.c_str(), 0, o.hashLoadFactor) != 0)
+            throw std::runtime_error(care_host_last_error());
+        std::printf("Saved minhasher\n");
+    }
     const double tBuilt = now_s();
     std::printf("Device: context and read store upload %.3fs, minhash tables %.3fs\n", tUploaded - tPacked,
                 tBuilt - tUploaded);

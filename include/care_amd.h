@@ -156,6 +156,48 @@ int care_stream_synchronize(care_context* ctx, struct ihipStream_t* stream);
  * include/gpu/gpuhashtable.cuh:743). */
 int care_minhash_build(care_context* ctx);
 
+/* One minhash map in key-grouped form: num_keys distinct keys (never
+ * UINT64_MAX), key i owning values[offsets[i] .. offsets[i] + counts[i]),
+ * the read ids that share that signature value.  This is the content of
+ * one CpuReadOnlyMultiValueHashTable (include/cpuhashtable.hpp:462-660):
+ * its values array and, per key, the ValueIndex (offset, count) of its
+ * lookup table. */
+typedef struct care_map_view {
+    uint64_t num_keys;
+    const uint64_t* keys;
+    const uint32_t* offsets;
+    const uint32_t* counts;
+    uint64_t num_values;
+    const uint32_t* values;
+} care_map_view;
+
+/* Installs num_maps prebuilt maps over the uploaded read store, in place of
+ * care_minhash_build (a hash-table file loaded by the host).  Counts must
+ * lie in 1..results_per_map (<= 65535), values below the number of reads,
+ * keys distinct within a map.  kmer_length and num_maps replace the
+ * context's k and number of maps; the signatures the queries use are
+ * computed on the device with them.
+ * Replaces OrdinaryCpuMinhasher/FakeGpuMinhasher::loadFromStream
+ * (include/ordinaryminhasher.hpp:318-339, include/gpu/fakegpuminhasher.cuh:505-525)
+ * after the host has parsed the file. */
+int care_minhash_import(care_context* ctx, const care_map_view* maps, int num_maps, int kmer_length,
+                        int results_per_map);
+
+/* k, number of maps and the per-key value limit (resultsPerMapThreshold)
+ * of the context's tables.  Replaces GpuMinhasher::getKmerSize /
+ * getNumberOfMaps / getNumResultsPerMapThreshold (include/gpu/gpuminhasher.cuh:90-94). */
+int care_minhash_geometry(care_context* ctx, int* kmer_length, int* num_maps, int* results_per_map);
+
+/* Size of map `map` in key-grouped form. */
+int care_minhash_map_size(care_context* ctx, int map, uint64_t* num_keys, uint64_t* num_values);
+/* Copies map `map` in key-grouped form, keys ascending and each key's
+ * values (ascending read ids) contiguous in key order, into caller arrays
+ * of the sizes care_minhash_map_size gives.  With care_minhash_map_size it
+ * replaces the table walk of writeToStream
+ * (include/ordinaryminhasher.hpp:303-316, include/cpuhashtable.hpp:221-232,627-636). */
+int care_minhash_export_map(care_context* ctx, int map, uint64_t* keys, uint32_t* offsets, uint32_t* counts,
+                            uint32_t* values);
+
 /* Minhash signatures of reads [first, first+n): n x num_hash_maps keys.
  * Replaces minhashSignatures3264Kernel (include/gpu/gpusequencehasher.cuh:116). */
 int care_minhash_signatures(care_context* ctx, uint64_t first, int n, uint64_t* out);

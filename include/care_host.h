@@ -43,6 +43,39 @@ const int32_t* care_readset_lengths(const care_readset* rs);
 const uint8_t* care_readset_ambiguous(const care_readset* rs);
 const uint32_t* care_readset_quals(const care_readset* rs);
 
+/* ---- Reference file formats ----------------------------------------------
+ * Files written here load in the reference's care-gpu and the other way
+ * round (same fields, packed in host byte order).  Functions returning int
+ * give 0 on success, 1 on error (see care_host_last_error). */
+
+/* Writes the read set as a preprocessed-reads file.  qual_bits is the
+ * bits-per-quality field written when the read set holds no qualities.
+ * Replaces ChunkedReadStorage::saveToFile (include/chunkedreadstorage.hpp:246-400). */
+int care_readset_save(const care_readset* rs, const char* path, int qual_bits);
+
+/* Reads a preprocessed-reads file.  With use_quality the file must hold
+ * qualities of qual_bits bits; without, its qualities are skipped.  Returns
+ * NULL on error.  Replaces ChunkedReadStorage::loadFromFile
+ * (include/chunkedreadstorage.hpp:158-243). */
+care_readset* care_readset_load_preprocessed(const char* path, int use_quality, int qual_bits);
+
+struct care_context;
+
+/* Writes the context's minhash tables as a hash-table file: k, the per-key
+ * value limit (results_per_map; 0 = the tables' own), the load factor of
+ * the written lookups (0 < load_factor <= 1) and every map.  Replaces
+ * FakeGpuMinhasher/OrdinaryCpuMinhasher::writeToStream
+ * (include/gpu/fakegpuminhasher.cuh:491-503, include/ordinaryminhasher.hpp:303-316). */
+int care_minhash_save_file(struct care_context* ctx, const char* path, int results_per_map, float load_factor);
+
+/* Loads the first min(max_maps, maps in the file) tables of a hash-table
+ * file into the context (after care_readstore_upload, in place of
+ * care_minhash_build); the file's k and map count then replace the
+ * context's.  *loaded_maps (may be NULL) gets the number loaded.  Replaces
+ * loadFromStream (include/gpu/fakegpuminhasher.cuh:505-525,
+ * src/gpu/gpuminhasherconstruction.cu:313-320). */
+int care_minhash_load_file(struct care_context* ctx, const char* path, int max_maps, int* loaded_maps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,5 +24,6 @@ mkdir -p "$out"
 g++ $FLAGS -shared -x c++ -o "$out/libcare_amd_emu.so" \
     "$root/care_amd/csrc/api.hip" "$root/care_amd/csrc/minhash.hip" "$root/care_amd/csrc/correct.hip" \
     "$root/care_amd/csrc/mlp.hip" "$root/care_amd/csrc/store.hip"
-g++ $HOSTFLAGS -o "$out/care-gpu-emu" "$root/care_amd/host/care_host.cpp" "$root/care_amd/host/care_gpu_main.cpp" \
+g++ $HOSTFLAGS -o "$out/care-gpu-emu" "$root/care_amd/host/care_host.cpp" "$root/care_amd/host/storage_files.cpp" \
+    "$root/care_amd/host/care_gpu_main.cpp" \
     -L"$out" -lcare_amd_emu -Wl,-rpath,"$out" -lz -lpthread

@@ -100,6 +100,7 @@ struct Sched {
     int cur = -1;
     int blockArrived = 0;
     unsigned long blockGen = 0;
+    int blockSite = 0;
     void (*body)(void*) = nullptr;
     void* arg = nullptr;
 };
@@ -142,10 +143,20 @@ inline void yield_to_scheduler() {
     if (_setjmp(f.jb) == 0) _longjmp(s->mainJb, 1);
 }
 
-inline void block_barrier() {
+// site: source line of the __syncthreads() call in checked builds.  The
+// threads of a block must meet at one call site: a barrier that pairs
+// different sites (one reached by a subset of the waves) is reported.
+inline void block_barrier(int site = 0) {
     Sched* s = sched();
     Fiber& f = s->fibers[s->cur];
     const unsigned long g = s->blockGen;
+    if (s->blockArrived == 0) {
+        s->blockSite = site;
+    } else if (site != s->blockSite) {
+        std::fprintf(stderr, "emu: threads of block %u meet at different barriers (lines %d and %d, thread %u)\n",
+                     ctx.bid.x, s->blockSite, site, ctx.tid.x);
+        std::abort();
+    }
     if (++s->blockArrived == int(s->fibers.size())) {
         s->blockArrived = 0;
         s->blockGen++;
@@ -373,7 +384,11 @@ inline void emu::check(bool ok, const char* what, const char* file, int line) {
 #define hipLaunchKernelGGL(kernel, grid, block, lds, stream, ...) \
     emu::launch_named(#kernel, kernel, grid, block, lds, __VA_ARGS__)
 
+#ifdef CARE_EMU_CHECKS
+#define __syncthreads() emu::block_barrier(__LINE__)
+#else
 inline void __syncthreads() { emu::block_barrier(); }
+#endif
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_wave_barrier() emu::wave_barrier()
 
