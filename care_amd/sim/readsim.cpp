@@ -191,6 +191,7 @@ int run_gen(int argc, char** argv) {
     std::uint64_t seed = 1;
     double genomeSize = -1, coverage = -1;
     int length = -1;
+    double nfrac = 0.0;  // paired-end profiles: fraction of reads given 1-3 N bases
     for (int i = 2; i < argc; i++) {
         std::string a = argv[i];
         auto val = [&]() -> std::string {
@@ -202,6 +203,7 @@ int run_gen(int argc, char** argv) {
         else if (a == "--genome") genomeSize = std::stod(val());
         else if (a == "--coverage") coverage = std::stod(val());
         else if (a == "--length") length = std::stoi(val());
+        else if (a == "--nfrac") nfrac = std::stod(val());
         else if (a == "--out") out = val();
         else if (a == "--truth") truthOut = val();
         else throw std::runtime_error("unknown argument " + a);
@@ -266,6 +268,14 @@ int run_gen(int argc, char** argv) {
                         if (q > 41) q = 41;
                         return char(33 + q);
                     });
+                }
+                if (nfrac > 0 && rng.uniform() < nfrac) {
+                    const int count = 1 + int(rng.below(3));
+                    for (int j = 0; j < count; j++) {
+                        const int npos = int(rng.below(L));
+                        read[npos] = 'N';
+                        qual[npos] = char(33 + 2);
+                    }
                 }
                 w.put("pair" + std::to_string(p) + "/" + std::to_string(m + 1), read, qual, tru);
             }

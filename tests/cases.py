@@ -30,6 +30,11 @@ CASES = {
     "se_forest_anchor": {"profile": "uniform-se", "seed": 107, "genome": 40000,
                          "args": ["-c", "30", "--pairmode", "SE", "-q", "-k", "20", "-h", "16", "--correctionType", "1"],
                          "forest": {"anchor": 3}},
+    # paired-end, fixed length, 2% of the reads with 1-3 Ns: the N rule,
+    # excluded ambiguous reads and 1-bit qualities on a paired-end input
+    "pe_ambig_q1": {"profile": "repeat-pe", "seed": 108, "genome": 60000, "sim_args": ["--nfrac", "0.02"],
+                    "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "1", "--excludeAmbiguous",
+                             "-k", "20", "-h", "32", "--candidateCorrection"]},
     # variable length without excluding ambiguous reads, 1-bit qualities
     "se_mixed_q1": {"profile": "mixed-se", "seed": 105, "genome": 80000,
                     "args": ["-c", "50", "--pairmode", "SE", "-q", "--qualityScoreBits", "1", "-k", "18", "-h", "24",

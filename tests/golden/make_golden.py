@@ -44,7 +44,8 @@ def main(write):
             reads = os.path.join(tmp, name + ".fastq")
             truth = os.path.join(tmp, name + ".truth.fastq")
             subprocess.run([READSIM, "gen", "--profile", case["profile"], "--seed", str(case["seed"]),
-                            "--genome", str(case["genome"]), "--out", reads, "--truth", truth], check=True)
+                            "--genome", str(case["genome"]), "--out", reads, "--truth", truth]
+                           + [str(a) for a in case.get("sim_args", [])], check=True)
             odir = os.path.join(tmp, name + "_out")
             subprocess.run([ORACLE, "-i", reads, "-o", "out.fastq", "-d", odir, "-t", "8"] + case_args(case, tmp),
                            check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)

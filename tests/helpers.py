@@ -32,7 +32,8 @@ def simulate(case, directory):
     truth = os.path.join(directory, "truth.fastq")
     if not os.path.exists(reads):
         subprocess.run([READSIM, "gen", "--profile", case["profile"], "--seed", str(case["seed"]),
-                        "--genome", str(case["genome"]), "--out", reads, "--truth", truth], check=True)
+                        "--genome", str(case["genome"]), "--out", reads, "--truth", truth]
+                       + [str(a) for a in case.get("sim_args", [])], check=True)
     return reads, truth
 
 

@@ -27,6 +27,10 @@ SMALL = {
     "se_mixed_small": {"profile": "mixed-se", "seed": 203, "genome": 2000,
                        "args": ["-c", "50", "--pairmode", "SE", "--excludeAmbiguous", "-q", "-k", "20", "-h", "48",
                                 "--candidateCorrection"]},
+    # paired-end with Ns (N rule, excluded ambiguous reads) and 1-bit qualities
+    "pe_ambig_small": {"profile": "repeat-pe", "seed": 206, "genome": 2000, "sim_args": ["--nfrac", "0.05"],
+                       "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "1", "--excludeAmbiguous",
+                                "-k", "20", "-h", "32", "--candidateCorrection"]},
     # reduced pipeline (top 20 candidates by map hits, no refinement) with
     # the classic decision
     "pe_reduced_small": {"profile": "repeat-pe", "seed": 205, "genome": 2000,
