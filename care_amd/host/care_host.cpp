@@ -18,6 +18,7 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -30,7 +31,10 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <fcntl.h>
+#include <sys/mman.h>
 #include <sys/stat.h>
+#include <unistd.h>
 #include <thread>
 #include <vector>
 
@@ -139,9 +143,40 @@ struct RecordReader {
     }
 };
 
+// Growable byte array whose new bytes are not zeroed: arenas filled by
+// bulk copies (memcpy, not element-wise) with no initialising pass.
+class Arena {
+public:
+    const char* data() const { return p_.get(); }
+    char* data() { return p_.get(); }
+    size_t size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+    void reserve(size_t c) {
+        if (c <= cap_) return;
+        std::unique_ptr<char[]> q(new char[c]);
+        if (n_) std::memcpy(q.get(), p_.get(), n_);
+        p_ = std::move(q);
+        cap_ = c;
+    }
+    void append(const char* b, size_t len) {
+        if (n_ + len > cap_) reserve(std::max(n_ + len, cap_ * 2 + 4096));
+        if (len) std::memcpy(p_.get() + n_, b, len);
+        n_ += len;
+    }
+    void resizeUninit(size_t m) {
+        reserve(m);
+        n_ = m;
+    }
+    bool operator==(const Arena& o) const { return n_ == o.n_ && (n_ == 0 || std::memcmp(data(), o.data(), n_) == 0); }
+
+private:
+    std::unique_ptr<char[]> p_;
+    size_t n_ = 0, cap_ = 0;
+};
+
 // All reads of a library in read-id order, stored in arenas.
 struct Library {
-    std::vector<char> headers, seqs, quals;
+    Arena headers, seqs, quals;
     std::vector<u64> hoff, soff;  // n + 1 offsets (quals share soff)
     std::vector<u32> fileOf;      // first read id of every input file, for the N rule batches
     bool fastq = true;
@@ -151,17 +186,217 @@ struct Library {
             hoff.push_back(0);
             soff.push_back(0);
         }
-        headers.insert(headers.end(), h.begin(), h.end());
+        headers.append(h.data(), h.size());
         hoff.push_back(headers.size());
-        seqs.insert(seqs.end(), s.begin(), s.end());
-        if (keepQual) quals.insert(quals.end(), q.begin(), q.end());
+        seqs.append(s.data(), s.size());
+        if (keepQual) quals.append(q.data(), q.size());
         soff.push_back(seqs.size());
     }
     const char* seq(u64 i) const { return seqs.data() + soff[i]; }
     int len(u64 i) const { return int(soff[i + 1] - soff[i]); }
 };
 
-Library readLibrary(const std::vector<std::string>& files, bool pairedEnd, bool keepQual) {
+// ---------------------------------------------------------------------------
+// Parallel parse of four-line FASTQ files (the usual layout).  The file is
+// mapped (or, gzip'ed, inflated into memory) and cut into one byte range
+// per thread; the threads count their newlines, a prefix sum gives every
+// range its first line number, and a record starts at each line whose
+// number is a multiple of four.  A range owns the records that start after
+// one of its newlines (range 0 also the first one), parses them into its
+// own arenas, and the arenas are copied into the library in parallel.  A
+// record that is not four lines ('@' header, sequence, '+' line, quality of
+// the sequence's length) makes the caller fall back to the sequential
+// parser above, which also handles FASTA and multi-line records; both give
+// the same library for four-line files.
+// ---------------------------------------------------------------------------
+struct InputBytes {
+    const char* p = nullptr;
+    size_t n = 0;
+    std::vector<char> inflated;
+    void* map = nullptr;
+    explicit InputBytes(const std::string& path) {
+        const int fd = open(path.c_str(), O_RDONLY);
+        if (fd < 0) throw std::runtime_error("cannot open input file " + path);
+        struct stat st {};
+        fstat(fd, &st);
+        unsigned char magic[2] = {0, 0};
+        const bool gz = pread(fd, magic, 2, 0) == 2 && magic[0] == 0x1f && magic[1] == 0x8b;
+        if (!gz && st.st_size > 0) {
+            map = mmap(nullptr, size_t(st.st_size), PROT_READ, MAP_PRIVATE, fd, 0);
+            close(fd);
+            if (map == MAP_FAILED) throw std::runtime_error("cannot map input file " + path);
+            p = static_cast<const char*>(map);
+            n = size_t(st.st_size);
+            return;
+        }
+        close(fd);
+        gzFile f = gzopen(path.c_str(), "rb");
+        if (!f) throw std::runtime_error("cannot open input file " + path);
+        gzbuffer(f, 1 << 20);
+        for (;;) {
+            const size_t have = inflated.size();
+            inflated.resize(have + (size_t(64) << 20));
+            const int r = gzread(f, inflated.data() + have, unsigned(size_t(64) << 20));
+            if (r < 0) {
+                gzclose(f);
+                throw std::runtime_error("read error on " + path);
+            }
+            inflated.resize(have + size_t(r));
+            if (r == 0) break;
+        }
+        gzclose(f);
+        p = inflated.data();
+        n = inflated.size();
+    }
+    ~InputBytes() {
+        if (map) munmap(map, n);
+    }
+    InputBytes(const InputBytes&) = delete;
+    InputBytes& operator=(const InputBytes&) = delete;
+};
+
+struct LibraryPart {
+    Arena headers, seqs, quals;
+    std::vector<u64> hend, send;  // end offsets of every record
+    bool ok = true;
+};
+
+// parses the records starting in [b, stop) (a record may run past stop)
+void parseRange(const char* b, const char* stop, const char* e, bool keepQual, LibraryPart& part) {
+    auto nextLine = [&](const char*& p, const char*& ls, const char*& le) {
+        ls = p;
+        const char* nl = static_cast<const char*>(std::memchr(p, '\n', size_t(e - p)));
+        le = nl ? nl : e;
+        p = nl ? nl + 1 : e;
+        if (le > ls && le[-1] == '\r') le--;
+    };
+    const char* p = b;
+    // the range's bytes bound every arena
+    const size_t bytes = size_t(std::min(stop, e) - b) + 1;
+    part.headers.reserve(bytes / 4);
+    part.seqs.reserve(bytes / 2);
+    if (keepQual) part.quals.reserve(bytes / 2);
+    part.hend.reserve(bytes / 64);
+    part.send.reserve(bytes / 64);
+    while (p < stop && p < e) {
+        const char *h0, *h1, *s0, *s1, *x0, *x1, *q0, *q1;
+        nextLine(p, h0, h1);
+        nextLine(p, s0, s1);
+        nextLine(p, x0, x1);
+        nextLine(p, q0, q1);
+        if (h1 == h0 || *h0 != '@' || x1 == x0 || *x0 != '+' || (q1 - q0) != (s1 - s0)) {
+            part.ok = false;
+            return;
+        }
+        part.headers.append(h0 + 1, size_t(h1 - h0 - 1));
+        part.hend.push_back(part.headers.size());
+        part.seqs.append(s0, size_t(s1 - s0));
+        part.send.push_back(part.seqs.size());
+        if (keepQual) part.quals.append(q0, size_t(q1 - q0));
+    }
+}
+
+// false: not a four-line FASTQ file (lib untouched)
+bool readFastq4(const std::string& path, bool keepQual, int threads, Library& out) {
+    InputBytes in(path);
+    if (in.n == 0 || in.p[0] != '@') return false;
+    const char* base = in.p;
+    const size_t n = in.n;
+    const int T = int(std::max<size_t>(1, std::min<size_t>(size_t(std::max(1, threads)), n >> 16)));
+    std::vector<size_t> lo(T + 1);
+    for (int t = 0; t <= T; t++) lo[t] = n * size_t(t) / size_t(T);
+    auto parallel = [&](auto&& fn) {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < T; t++) pool.emplace_back(fn, t);
+        fn(0);
+        for (auto& th : pool) th.join();
+    };
+    // newlines per range, then the first line number of every range
+    std::vector<u64> nl(T + 1, 0);
+    parallel([&](int t) { nl[t + 1] = u64(std::count(base + lo[t], base + lo[t + 1], '\n')); });
+    for (int t = 0; t < T; t++) nl[t + 1] += nl[t];
+    // first record start owned by each range
+    std::vector<const char*> first(T + 1, base + n);
+    parallel([&](int t) {
+        if (t == 0) {
+            first[0] = base;
+            return;
+        }
+        u64 k = nl[t];  // newlines before this range
+        for (const char* q = base + lo[t]; q < base + lo[t + 1]; q++) {
+            if (*q != '\n') continue;
+            k++;  // the line after this newline has number k
+            if (k % 4 == 0) {
+                first[t] = q + 1;
+                return;
+            }
+        }
+    });
+    // a range parses from its first record to the next range's first record
+    for (int t = T - 1; t >= 0; t--) first[t] = std::min(first[t], first[t + 1]);
+    std::vector<LibraryPart> parts(T);
+    parallel([&](int t) { parseRange(first[t], first[t + 1], base + n, keepQual, parts[t]); });
+    for (auto& part : parts)
+        if (!part.ok) return false;
+    // join in range order, each range copying its own arenas
+    if (out.hoff.empty()) {
+        out.hoff.push_back(0);
+        out.soff.push_back(0);
+    }
+    std::vector<u64> h0(T + 1), s0(T + 1), q0(T + 1), r0(T + 1);
+    h0[0] = out.headers.size();
+    s0[0] = out.seqs.size();
+    q0[0] = out.quals.size();
+    r0[0] = out.size();
+    for (int t = 0; t < T; t++) {
+        h0[t + 1] = h0[t] + parts[t].headers.size();
+        s0[t + 1] = s0[t] + parts[t].seqs.size();
+        q0[t + 1] = q0[t] + parts[t].quals.size();
+        r0[t + 1] = r0[t] + parts[t].hend.size();
+    }
+    out.headers.resizeUninit(h0[T]);
+    out.seqs.resizeUninit(s0[T]);
+    out.quals.resizeUninit(q0[T]);
+    out.hoff.resize(r0[T] + 1);
+    out.soff.resize(r0[T] + 1);
+    parallel([&](int t) {
+        LibraryPart& part = parts[t];
+        if (!part.headers.empty()) std::memcpy(out.headers.data() + h0[t], part.headers.data(), part.headers.size());
+        if (!part.seqs.empty()) std::memcpy(out.seqs.data() + s0[t], part.seqs.data(), part.seqs.size());
+        if (!part.quals.empty()) std::memcpy(out.quals.data() + q0[t], part.quals.data(), part.quals.size());
+        for (size_t i = 0; i < part.hend.size(); i++) {
+            out.hoff[r0[t] + 1 + i] = h0[t] + part.hend[i];
+            out.soff[r0[t] + 1 + i] = s0[t] + part.send[i];
+        }
+        part = LibraryPart();
+    });
+    out.fastq = true;
+    return true;
+}
+
+// Mates of two libraries interleaved, up to the shorter one.
+Library interleave(const Library& a, const Library& b) {
+    Library lib;
+    lib.hoff.push_back(0);
+    lib.soff.push_back(0);
+    lib.fileOf.push_back(0);
+    const u64 n = std::min(a.size(), b.size());
+    auto add = [&](const Library& x, u64 i) {
+        lib.headers.append(x.headers.data() + x.hoff[i], x.hoff[i + 1] - x.hoff[i]);
+        lib.hoff.push_back(lib.headers.size());
+        lib.seqs.append(x.seqs.data() + x.soff[i], x.soff[i + 1] - x.soff[i]);
+        if (!x.quals.empty()) lib.quals.append(x.quals.data() + x.soff[i], x.soff[i + 1] - x.soff[i]);
+        lib.soff.push_back(lib.seqs.size());
+    };
+    for (u64 i = 0; i < n; i++) {
+        add(a, i);
+        add(b, i);
+    }
+    lib.fastq = a.fastq;
+    return lib;
+}
+
+Library readLibrarySequential(const std::vector<std::string>& files, bool pairedEnd, bool keepQual) {
     Library lib;
     std::string h1, s1, q1, h2, s2, q2;
     if (pairedEnd && files.size() == 2) {
@@ -189,6 +424,36 @@ Library readLibrary(const std::vector<std::string>& files, bool pairedEnd, bool 
     return lib;
 }
 
+// All reads of the inputs in read-id order (include/readlibraryio.hpp:130-190):
+// four-line FASTQ files in parallel, anything else sequentially.
+Library readLibrary(const std::vector<std::string>& files, bool pairedEnd, bool keepQual, int threads) {
+    if (threads > 1) {
+        if (pairedEnd && files.size() == 2) {
+            Library a, b;
+            if (readFastq4(files[0], keepQual, threads, a) && readFastq4(files[1], keepQual, threads, b))
+                return interleave(a, b);
+        } else {
+            Library lib;
+            bool ok = true;
+            for (const auto& f : files) {
+                lib.fileOf.push_back(u32(lib.size()));
+                if (!readFastq4(f, keepQual, threads, lib)) {
+                    ok = false;
+                    break;
+                }
+            }
+            if (ok) {
+                if (lib.hoff.empty()) {
+                    lib.hoff.push_back(0);
+                    lib.soff.push_back(0);
+                }
+                return lib;
+            }
+        }
+    }
+    return readLibrarySequential(files, pairedEnd, keepQual);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -197,32 +462,28 @@ Library readLibrary(const std::vector<std::string>& files, bool pairedEnd, bool 
 
 namespace {
 
-inline u32 code2(char c) {
-    switch (c) {
-        case 'C': return 1;
-        case 'G': return 2;
-        case 'T': return 3;
-        default: return 0;
-    }
-}
-
 void encodeQualityRow(u32* out, const char* q, int len, int bits) {
     if (bits == 8) {
         std::memcpy(out, q, size_t(len));
         return;
     }
+    // bins of every character: Q = char - 33, bin edges 13 (1 bit) and 4, 13, 20 (2 bits)
+    static const std::array<std::array<u8, 256>, 2> kBins = [] {
+        std::array<std::array<u8, 256>, 2> t{};
+        for (int c = 0; c < 256; c++) {
+            const int Q = int(static_cast<signed char>(c)) - 33;
+            t[0][size_t(c)] = Q <= 13 ? 0 : 1;
+            t[1][size_t(c)] = Q <= 4 ? 0 : Q <= 13 ? 1 : Q <= 20 ? 2 : 3;
+        }
+        return t;
+    }();
+    const u8* bins = kBins[size_t(bits - 1)].data();
     const int per = 32 / bits;
     const int words = (bits * len + 31) / 32;
     for (int w = 0; w < words; w++) {
         u32 data = 0;
         const int e = std::min(len, (w + 1) * per);
-        for (int p = w * per; p < e; p++) {
-            const int Q = int(q[p]) - 33;
-            u32 bin;
-            if (bits == 1) bin = Q <= 13 ? 0 : 1;
-            else bin = Q <= 4 ? 0 : Q <= 13 ? 1 : Q <= 20 ? 2 : 3;
-            data = (data << bits) | bin;
-        }
+        for (int p = w * per; p < e; p++) data = (data << bits) | bins[static_cast<unsigned char>(q[p])];
         if (w == words - 1) data <<= ((w + 1) * per - len) * bits;
         out[w] = data;
     }
@@ -250,8 +511,17 @@ std::unique_ptr<care_readset> packLibrary(const Library& lib, int qualBits, int 
     for (size_t f = 0; f + 1 < starts.size(); f++)
         for (u64 b = starts[f]; b < starts[f + 1]; b += 65536) batches.emplace_back(b, std::min(starts[f + 1], b + 65536));
     std::atomic<size_t> next{0};
+    // 2-bit codes of the characters, either case; 4 = anything else
+    static const std::array<u8, 256> kCode = [] {
+        std::array<u8, 256> t{};
+        t.fill(4);
+        t['A'] = t['a'] = 0;
+        t['C'] = t['c'] = 1;
+        t['G'] = t['g'] = 2;
+        t['T'] = t['t'] = 3;
+        return t;
+    }();
     auto work = [&]() {
-        std::vector<char> tmp;
         for (;;) {
             const size_t bi = next.fetch_add(1);
             if (bi >= batches.size()) break;
@@ -259,23 +529,24 @@ std::unique_ptr<care_readset> packLibrary(const Library& lib, int qualBits, int 
             for (u64 i = batches[bi].first; i < batches[bi].second; i++) {
                 const int len = lib.len(i);
                 const char* s = lib.seq(i);
-                tmp.assign(s, s + len);
                 bool amb = false;
-                for (auto& c : tmp) {
-                    if (c == 'a') c = 'A';
-                    else if (c == 'c') c = 'C';
-                    else if (c == 'g') c = 'G';
-                    else if (c == 't') c = 'T';
-                    else if (c != 'A' && c != 'C' && c != 'G' && c != 'T') {
-                        c = "ACGT"[ncount];
-                        ncount = (ncount + 1) & 3;
-                        amb = true;
+                u32* row = rs->seqs.data() + i * rs->pitch;
+                for (int w0 = 0; w0 < len; w0 += 16) {
+                    const int e = std::min(16, len - w0);
+                    u32 word = 0;
+                    for (int j = 0; j < e; j++) {
+                        u32 c = kCode[static_cast<unsigned char>(s[w0 + j])];
+                        if (c > 3) {
+                            c = u32(ncount);
+                            ncount = (ncount + 1) & 3;
+                            amb = true;
+                        }
+                        word |= c << (30 - 2 * j);
                     }
+                    row[w0 >> 4] = word;
                 }
                 rs->ambig[i] = amb;
                 rs->lens[i] = len;
-                u32* row = rs->seqs.data() + i * rs->pitch;
-                for (int p = 0; p < len; p++) row[p >> 4] |= code2(tmp[p]) << (30 - 2 * (p & 15));
                 if (qualBits) encodeQualityRow(rs->quals.data() + i * rs->qpitch, lib.quals.data() + lib.soff[i], len, qualBits);
             }
         }
@@ -614,7 +885,7 @@ int runMain(int argc, char** argv) {
     // STEP 1
     std::printf("STEP 1: Database construction\n");
     const double t0 = now_s();
-    Library lib = readLibrary(o.inputs, o.eng.paired_end != 0, true);
+    Library lib = readLibrary(o.inputs, o.eng.paired_end != 0, true, threads);
     if (!lib.fastq) o.eng.use_quality = 0;
     const double tParsed = now_s();
     std::unique_ptr<care_readset> rs;
@@ -934,7 +1205,8 @@ care_readset* care_readset_load(const char* const* files, int num_files, int pai
         if (qual_bits != 0 && qual_bits != 1 && qual_bits != 2 && qual_bits != 8)
             throw std::runtime_error("qual_bits must be 0, 1, 2 or 8");
         std::vector<std::string> f(files, files + num_files);
-        Library lib = readLibrary(f, paired_end != 0, qual_bits != 0);
+        const unsigned hcp = std::thread::hardware_concurrency();
+        Library lib = readLibrary(f, paired_end != 0, qual_bits != 0, int(hcp ? std::min(hcp, 16u) : 4u));
         if (!lib.fastq) qual_bits = 0;
         const unsigned hc = std::thread::hardware_concurrency();
         return packLibrary(lib, qual_bits, int(hc ? std::min(hc, 16u) : 4u)).release();
