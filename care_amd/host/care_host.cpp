@@ -575,6 +575,8 @@ struct HostOptions {
     bool autotune = true;  // time launch shapes on a sample of anchors before STEP 2
     int candResultsPerAnchor = 64;  // room for candidate corrections per anchor of a chunk
     float hashLoadFactor = 0.8f;    // --hashloadfactor: load of the lookups in a written hash-table file
+    u64 memTotal = 0;               // -m/--memTotal in bytes (0: not given)
+    std::string tempdir;            // --tempdir (default: the output directory)
     std::string forestAnchor, forestCands;
     // checkpoint files of the reference (preprocessed reads, hash tables)
     std::string saveReads, loadReads, saveTables, loadTables;
@@ -594,6 +596,21 @@ std::vector<int> parseIntList(const std::string& s) {
         p = e + 1;
     }
     return v;
+}
+
+// byte count with an optional K, M or G suffix (src/options.cpp:239-265)
+u64 parseMemory(const std::string& v) {
+    if (v.empty()) return 0;
+    u64 factor = 1;
+    std::string digits = v;
+    switch (v.back()) {
+        case 'K': factor = u64(1) << 10; break;
+        case 'M': factor = u64(1) << 20; break;
+        case 'G': factor = u64(1) << 30; break;
+        default: break;
+    }
+    if (factor != 1) digits.pop_back();
+    return factor * std::stoull(digits);
 }
 
 HostOptions parseOptions(int argc, char** argv) {
@@ -680,9 +697,11 @@ HostOptions parseOptions(int argc, char** argv) {
             if (!(o.hashLoadFactor > 0.f && o.hashLoadFactor <= 1.f))
                 throw std::runtime_error("--hashloadfactor must lie in (0, 1]");
         }
-        // memory limits, the temporary directory and the reference's table
+        else if (a == "-m" || a == "--memTotal") o.memTotal = parseMemory(value());
+        else if (a == "--tempdir") o.tempdir = value();
+        // the host memory for hash tables and the reference's table
         // implementation choices do not apply to this build's device tables
-        else if (a == "-m" || a == "--memTotal" || a == "--memHashtables" || a == "--tempdir" ||
+        else if (a == "--memHashtables" ||
                  a == "--warpcore" || a == "--replicateGpuData" || a == "--singlehash" ||
                  a == "--samplingRateAnchor" || a == "--samplingRateCands")
             (void)value();
@@ -711,13 +730,99 @@ struct PinnedArray {
 // ---------------------------------------------------------------------------
 // Correction results (include/correctedsequence.hpp:22-146 semantics)
 // ---------------------------------------------------------------------------
+// Result bytes held in memory up to a limit and appended to a temporary
+// file in the temporary directory beyond it, the file mapped back for
+// STEP 3 (what the reference's SerializedObjectStorage over a
+// FileBackedUVector does with -m/--memTotal and --tempdir,
+// include/serializedobjectstorage.hpp:15-80).  The file is unlinked as soon
+// as it is created, so it disappears with the process.
+class SpillBytes {
+public:
+    SpillBytes() = default;
+    SpillBytes(const SpillBytes&) = delete;
+    SpillBytes& operator=(const SpillBytes&) = delete;
+    SpillBytes(SpillBytes&& o) noexcept { *this = std::move(o); }
+    SpillBytes& operator=(SpillBytes&& o) noexcept {
+        if (this != &o) {
+            release();
+            mem_ = std::move(o.mem_);
+            limit_ = o.limit_;
+            dir_ = std::move(o.dir_);
+            fd_ = o.fd_;
+            fileBytes_ = o.fileBytes_;
+            map_ = o.map_;
+            o.fd_ = -1;
+            o.fileBytes_ = 0;
+            o.map_ = nullptr;
+        }
+        return *this;
+    }
+    ~SpillBytes() { release(); }
+    void configure(u64 limit, const std::string& dir) {
+        limit_ = limit;
+        dir_ = dir;
+    }
+    u64 size() const { return fileBytes_ + mem_.size(); }
+    u64 spilledBytes() const { return fileBytes_; }
+    void push_back(char c) { mem_.push_back(c); }
+    void append(const char* p, size_t n) { mem_.insert(mem_.end(), p, p + n); }
+    // back to n bytes; n never reaches into the spilled part (records are
+    // settled whole)
+    void truncate(u64 n) { mem_.resize(size_t(n - fileBytes_)); }
+    // between records: beyond the limit the bytes in memory go to the file
+    void settle() {
+        if (mem_.size() > limit_) spill();
+    }
+    // all bytes, contiguous: memory, or the mapped file
+    const char* finish() {
+        if (fd_ < 0) return mem_.data();
+        if (!map_) {
+            spill();
+            void* m = mmap(nullptr, size_t(fileBytes_), PROT_READ, MAP_PRIVATE, fd_, 0);
+            if (m == MAP_FAILED) throw std::runtime_error("cannot map the spilled results");
+            map_ = static_cast<const char*>(m);
+        }
+        return map_;
+    }
+
+private:
+    void spill() {
+        if (fd_ < 0) {
+            std::string path = dir_ + "/care_results_XXXXXX";
+            fd_ = mkstemp(&path[0]);
+            if (fd_ < 0) throw std::runtime_error("cannot create a temporary file in " + dir_);
+            unlink(path.c_str());
+        }
+        size_t done = 0;
+        while (done < mem_.size()) {
+            const ssize_t w = write(fd_, mem_.data() + done, mem_.size() - done);
+            if (w <= 0) throw std::runtime_error("write error on a temporary file in " + dir_);
+            done += size_t(w);
+        }
+        fileBytes_ += mem_.size();
+        mem_.clear();
+    }
+    void release() {
+        if (map_) munmap(const_cast<char*>(map_), size_t(fileBytes_));
+        if (fd_ >= 0) close(fd_);
+        map_ = nullptr;
+        fd_ = -1;
+    }
+    std::vector<char> mem_;
+    u64 limit_ = ~u64(0);
+    std::string dir_ = ".";
+    int fd_ = -1;
+    u64 fileBytes_ = 0;
+    const char* map_ = nullptr;
+};
+
 struct ResultStore {
     // one entry per stored result
     std::vector<u32> readId;
     std::vector<u8> flags;      // bit0 anchor, bit1 hq, bit2 useEdits
     std::vector<u64> dataOff;   // offset into data (edits: pos u16 + base; or full sequence)
     std::vector<u32> dataLen;   // number of edits or sequence length
-    std::vector<char> data;
+    SpillBytes data;
 
     void add(u32 id, bool anchor, bool hq, const char* corrected, const char* original, int len, bool ambiguous) {
         // include/corrector.hpp:1645-1668 and 1717-1757
@@ -738,51 +843,46 @@ struct ResultStore {
             }
             useEdits = edits <= maxEdits;
             if (useEdits) cnt = u32(edits);
-            else data.resize(off);
+            else data.truncate(off);
         }
         if (useEdits) {
             // HQ anchors without edits are not stored (src/correct_cpu.cpp:262-266)
             if (anchor && hq && cnt == 0) return;
             f |= 4;
         } else {
-            data.insert(data.end(), corrected, corrected + len);
+            data.append(corrected, size_t(len));
             cnt = u32(len);
         }
         readId.push_back(id);
         flags.push_back(f);
         dataOff.push_back(off);
         dataLen.push_back(cnt);
+        data.settle();
     }
 };
 
-// One store holding the entries of `parts` in order (data offsets shifted).
-ResultStore mergeStores(std::vector<ResultStore>& parts) {
-    ResultStore all;
-    size_t n = 0, bytes = 0;
-    for (const auto& p : parts) {
-        n += p.readId.size();
-        bytes += p.data.size();
+// Entries of several stores, each with a pointer to its bytes.
+struct ResultView {
+    std::vector<u32> readId;
+    std::vector<u8> flags;
+    std::vector<u32> dataLen;
+    std::vector<const char*> ptr;
+    void append(ResultStore& s) {
+        const char* base = s.data.finish();
+        readId.insert(readId.end(), s.readId.begin(), s.readId.end());
+        flags.insert(flags.end(), s.flags.begin(), s.flags.end());
+        dataLen.insert(dataLen.end(), s.dataLen.begin(), s.dataLen.end());
+        for (u64 off : s.dataOff) ptr.push_back(base + off);
+        std::vector<u32>().swap(s.readId);
+        std::vector<u8>().swap(s.flags);
+        std::vector<u32>().swap(s.dataLen);
+        std::vector<u64>().swap(s.dataOff);
     }
-    all.readId.reserve(n);
-    all.flags.reserve(n);
-    all.dataOff.reserve(n);
-    all.dataLen.reserve(n);
-    all.data.reserve(bytes);
-    for (auto& p : parts) {
-        const u64 base = all.data.size();
-        all.readId.insert(all.readId.end(), p.readId.begin(), p.readId.end());
-        all.flags.insert(all.flags.end(), p.flags.begin(), p.flags.end());
-        all.dataLen.insert(all.dataLen.end(), p.dataLen.begin(), p.dataLen.end());
-        for (u64 off : p.dataOff) all.dataOff.push_back(off + base);
-        all.data.insert(all.data.end(), p.data.begin(), p.data.end());
-        p = ResultStore();
-    }
-    return all;
-}
+};
 
 // src/correctionresultoutput.cpp:730-1023 (combineMultipleCorrectionResults1_rawtcs2);
 // returns label 0 uncorrected, 1 lq anchor only, 2 lq with candidates, 3 hq
-int combineRead(const ResultStore& rsx, const std::vector<u64>& entries, std::string& seq) {
+int combineRead(const ResultView& rsx, const std::vector<u64>& entries, std::string& seq) {
     if (entries.empty()) return 0;
     long anchorIdx = -1;
     for (u64 e : entries)
@@ -793,13 +893,13 @@ int combineRead(const ResultStore& rsx, const std::vector<u64>& entries, std::st
     if (anchorIdx < 0) return 0;
     auto apply = [&](u64 e, std::string& s) {
         if (rsx.flags[e] & 4) {
-            const char* d = rsx.data.data() + rsx.dataOff[e];
+            const char* d = rsx.ptr[e];
             for (u32 k = 0; k < rsx.dataLen[e]; k++) {
                 const int pos = int((unsigned char)d[3 * k]) | (int((unsigned char)d[3 * k + 1]) << 8);
                 s[size_t(pos)] = d[3 * k + 2];
             }
         } else {
-            s.assign(rsx.data.data() + rsx.dataOff[e], rsx.dataLen[e]);
+            s.assign(rsx.ptr[e], rsx.dataLen[e]);
         }
     };
     const u64 a = u64(anchorIdx);
@@ -817,10 +917,10 @@ int combineRead(const ResultStore& rsx, const std::vector<u64>& entries, std::st
             const bool e0 = rsx.flags[f0] & 4, e1 = rsx.flags[e] & 4;
             if (e0 && e1) {
                 eq = rsx.dataLen[f0] == rsx.dataLen[e] &&
-                     std::memcmp(rsx.data.data() + rsx.dataOff[f0], rsx.data.data() + rsx.dataOff[e], 3 * size_t(rsx.dataLen[e])) == 0;
+                     std::memcmp(rsx.ptr[f0], rsx.ptr[e], 3 * size_t(rsx.dataLen[e])) == 0;
             } else if (!e0 && !e1) {
                 eq = rsx.dataLen[f0] == rsx.dataLen[e] &&
-                     std::memcmp(rsx.data.data() + rsx.dataOff[f0], rsx.data.data() + rsx.dataOff[e], rsx.dataLen[e]) == 0;
+                     std::memcmp(rsx.ptr[f0], rsx.ptr[e], rsx.dataLen[e]) == 0;
             } else {
                 std::string other = seq;
                 apply(e, other);
@@ -877,9 +977,11 @@ int runMain(int argc, char** argv) {
     if (o.eng.paired_end && o.inputs.size() > 2) throw std::runtime_error("paired-end input takes one or two files");
     if (o.eng.use_quality && o.qualityBits != 1 && o.qualityBits != 2 && o.qualityBits != 8)
         throw std::runtime_error("qualityScoreBits must be 1, 2 or 8");
-    // output directory and its parents (std::filesystem::create_directories)
-    for (size_t i = 1; i <= o.outdir.size(); i++)
-        if (i == o.outdir.size() || o.outdir[i] == '/') mkdir(o.outdir.substr(0, i).c_str(), 0755);
+    // output and temporary directories and their parents (std::filesystem::create_directories)
+    if (o.tempdir.empty()) o.tempdir = o.outdir;
+    for (const std::string* dir : {&o.outdir, &o.tempdir})
+        for (size_t i = 1; i <= dir->size(); i++)
+            if (i == dir->size() || (*dir)[i] == '/') mkdir(dir->substr(0, i).c_str(), 0755);
     const int threads = std::max(1, o.threads);
 
     // STEP 1
@@ -995,6 +1097,16 @@ int runMain(int argc, char** argv) {
     const double t1 = now_s();
     std::vector<u8> hqAnchor(rs->n, 0);
     std::vector<ResultStore> anchorParts(static_cast<size_t>(threads)), candParts(static_cast<size_t>(threads));
+    {
+        // -m: what the library and the packed reads leave of the host memory
+        // limit holds the results; beyond it they go to --tempdir
+        const u64 held = lib.headers.size() + lib.seqs.size() + lib.quals.size() +
+                         8 * (lib.hoff.size() + lib.soff.size()) + 4 * (rs->seqs.size() + rs->quals.size()) +
+                         5 * rs->n;
+        const u64 perStore = o.memTotal ? (o.memTotal > held ? (o.memTotal - held) / u64(2 * threads) : 0) : ~u64(0);
+        for (auto& p : anchorParts) p.data.configure(perStore, o.tempdir);
+        for (auto& p : candParts) p.data.configure(perStore, o.tempdir);
+    }
     auto decode = [&](u64 id, std::string& out) {
         const int len = rs->lens[id];
         out.resize(size_t(len));
@@ -1109,7 +1221,18 @@ int runMain(int argc, char** argv) {
     }
     device.join();
     if (!deviceError.empty()) throw std::runtime_error("engine: " + deviceError);
-    ResultStore anchors = mergeStores(anchorParts), cands = mergeStores(candParts);
+    ResultView anchors, cands;
+    u64 spilled = 0;
+    for (auto& p : anchorParts) {
+        anchors.append(p);
+        spilled += p.data.spilledBytes();
+    }
+    for (auto& p : candParts) {
+        cands.append(p);
+        spilled += p.data.spilledBytes();
+    }
+    if (spilled) std::printf("Correction results: %llu bytes kept in temporary files in %s\n",
+                             (unsigned long long)spilled, o.tempdir.c_str());
     const double t2 = now_s();
     std::printf("# elapsed time (STEP2): %.6fs\n", t2 - t1);
     std::printf("Correction throughput : ~%.1f reads/second.\n", double(numAnchors) / (t2 - t1));
@@ -1131,17 +1254,12 @@ int runMain(int argc, char** argv) {
     for (u64 e = 0; e < na; e++) entry[fill[anchors.readId[e]]++] = e;
     for (u64 e = 0; e < cands.readId.size(); e++)
         if (!hqAnchor[cands.readId[e]]) entry[fill[cands.readId[e]]++] = na + e;
-    ResultStore all;
-    all.readId = anchors.readId;
+    ResultView all = std::move(anchors);
     all.readId.insert(all.readId.end(), cands.readId.begin(), cands.readId.end());
-    all.flags = anchors.flags;
     all.flags.insert(all.flags.end(), cands.flags.begin(), cands.flags.end());
-    all.dataLen = anchors.dataLen;
     all.dataLen.insert(all.dataLen.end(), cands.dataLen.begin(), cands.dataLen.end());
-    all.dataOff = anchors.dataOff;
-    for (u64 off : cands.dataOff) all.dataOff.push_back(off + anchors.data.size());
-    all.data.swap(anchors.data);
-    all.data.insert(all.data.end(), cands.data.begin(), cands.data.end());
+    all.ptr.insert(all.ptr.end(), cands.ptr.begin(), cands.ptr.end());
+    cands = ResultView();
     long long lastWithResult = -1;
     for (u64 i = 0; i < rs->n; i++)
         if (count[i]) lastWithResult = (long long)i;

@@ -333,6 +333,25 @@ def test_emulated_load_errors(emu_exe, tmp_path):
     assert p.returncode != 0 and "preprocessed reads file holds" in p.stdout
 
 
+def test_emulated_spilled_results(emu_exe, tmp_path):
+    """-m below what the reads already take: every correction result goes
+    to a temporary file in --tempdir, which is gone after the run, and the
+    output is unchanged."""
+    d = str(tmp_path)
+    reads, _ = helpers.simulate(SE_CASE, d)
+    p, out1 = _run(emu_exe, reads, d, SE_ARGS + ["--candidateCorrection"], [], "a.fastq")
+    assert p.returncode == 0, p.stdout[-3000:]
+    assert "temporary files" not in p.stdout
+    tmp = os.path.join(d, "spill")
+    for mem in ("1K", "300K"):
+        p, out2 = _run(emu_exe, reads, d, SE_ARGS + ["--candidateCorrection"], ["-m", mem, "--tempdir", tmp],
+                       "b%s.fastq" % mem)
+        assert p.returncode == 0, p.stdout[-3000:]
+        assert "kept in temporary files in " + tmp in p.stdout
+        assert helpers.md5(out2) == helpers.md5(out1)
+        assert os.listdir(tmp) == []
+
+
 def test_engine_map_export_import_emulated(tmp_path):
     """care_minhash_export_map gives each map in key order; importing the
     exported maps into a fresh context reproduces every candidate list."""
