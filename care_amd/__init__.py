@@ -146,6 +146,8 @@ ENGINE_SYMBOLS = {
                                           ctypes.POINTER(BatchResult)]),
     "care_correct_batch_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
                                                   ctypes.POINTER(BatchResult)]),
+    "care_correct_batch_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                                ctypes.POINTER(BatchResult)]),
     "care_set_launch_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "care_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CareTuning)]),
     "care_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CareTuning)]),
@@ -461,11 +463,13 @@ class Engine:
             self._ck(rc)
             return ids[: offs[n]], offs
 
-    def correct(self, first, n, candidate_capacity=None, out=None, noalloc=False):
+    def correct(self, first, n, candidate_capacity=None, out=None, noalloc=False, graph=False):
         """Corrects anchors [first, first+n).  out: arrays reused across
         calls, (status, seqs) or (status, seqs, cand_ids, cand_seqs):
         uint8 [n], uint8 [n][pitch], uint32 [cap], uint8 [cap][pitch].
-        noalloc: through care_correct_batch_noalloc (no device allocation)."""
+        noalloc: through care_correct_batch_noalloc (no device allocation);
+        graph: through care_correct_batch_graph (out in page-locked memory,
+        pinned_empty)."""
         pitch = (self.max_length + 15) // 16 * 16
         ccap = candidate_capacity if candidate_capacity is not None else (n * 64 if self.options.correct_candidates else 0)
         if out is not None:
@@ -488,7 +492,8 @@ class Engine:
         r.candidate_read_ids = cids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
         r.candidate_sequences = cseqs.ctypes.data_as(ctypes.POINTER(ctypes.c_char))
         r.candidate_capacity = ccap
-        fn = self.lib.care_correct_batch_noalloc if noalloc else self.lib.care_correct_batch
+        fn = (self.lib.care_correct_batch_noalloc if noalloc else
+              self.lib.care_correct_batch_graph if graph else self.lib.care_correct_batch)
         self._ck(fn(self.ctx, first, n, ctypes.byref(r)))
         nc = r.num_candidates
         return status, seqs, cids[:nc], cseqs[:nc]

@@ -174,6 +174,23 @@ struct care_context {
 
     care_batch_stats stats{};
 
+    // care_correct_batch_graph: the device work of the last call, captured
+    // into a graph and replayed while calls repeat it (same range, outputs,
+    // device buffers, options and launch shapes)
+    struct GraphCache {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<unsigned char> key;
+        int launches = 0;
+        void release() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+            key.clear();
+        }
+    } graphCache;
+
     struct ForestStore {
         DevBuf<ForestNode> nodes;
         DevBuf<u32> treeStart;
@@ -344,6 +361,7 @@ int care_context_destroy(care_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->copyStream) (void)hipStreamSynchronize(ctx->copyStream);
+    ctx->graphCache.release();
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->batchEv) (void)hipEventDestroy(e);
@@ -985,7 +1003,7 @@ int care_minhash_query(care_context* ctx, uint64_t first, int n, uint32_t* ids, 
 }  // extern "C"
 
 namespace {
-int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r);
+int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r, bool graph);
 
 // rows of `width` bytes from a device buffer with pitch spitch to host rows
 // with pitch dpitch: one contiguous copy when the pitches agree
@@ -1013,7 +1031,11 @@ bool is_page_locked(const void* p) {
 }
 
 extern "C" int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
-    return correct_batch_impl(ctx, first, n, r);
+    return correct_batch_impl(ctx, first, n, r, false);
+}
+
+extern "C" int care_correct_batch_graph(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
+    return correct_batch_impl(ctx, first, n, r, true);
 }
 
 namespace {
@@ -1093,7 +1115,17 @@ int prepare_params(care_context* ctx, CorrectParams& P) {
     return CARE_OK;
 }
 
-int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r) {
+// bytes of every value in `vals`, appended to key
+template <class... T>
+void key_append(std::vector<unsigned char>& key, const T&... vals) {
+    auto one = [&](const auto& v) {
+        const auto* b = reinterpret_cast<const unsigned char*>(&v);
+        key.insert(key.end(), b, b + sizeof(v));
+    };
+    (one(vals), ...);
+}
+
+int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r, bool graph) {
     if (!ctx || !r || !r->anchor_status || !r->anchor_sequences) return fail(CARE_ERR_ARGUMENT, "null argument");
     if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
     if (n < 0 || first + uint64_t(n) > ctx->rs.n) return fail(CARE_ERR_ARGUMENT, "range outside read store");
@@ -1162,50 +1194,94 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
         // pageable ones once at the end (a pageable copy would stall the
         // host between batches)
         const bool overlapCopies = is_page_locked(r->anchor_status) && is_page_locked(r->anchor_sequences);
-        launch_fill_u64(reinterpret_cast<u64*>(ctx->rawHits.p), 2, 0, s);  // totals over the call
-        launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);  // candidate results over the call
+        // graph mode: the device work below (kernels, the anchor result
+        // copies into the caller's page-locked arrays and the counter copies)
+        // is captured once and replayed while the calls repeat it; the
+        // per-kernel timing events are left out of the graph
+        auto& gc = ctx->graphCache;
+        std::vector<unsigned char> key;
+        if (graph) {
+            if (!overlapCopies)
+                return fail(CARE_ERR_ARGUMENT, "care_correct_batch_graph needs page-locked anchor_status and "
+                                               "anchor_sequences (care_host_buffer_alloc)");
+            key_append(key, first, n, r->anchor_status, r->anchor_sequences, r->sequence_pitch, candCap, B, T,
+                       blocksPerCU, ctx->msaChunk, ctx->queryCap, ctx->queryRegMax, ctx->numCUs, ctx->outPitch, o,
+                       P, ctx->tables, ctx->rs, ctx->candIds.p, ctx->candCount.p, ctx->metaA.p, ctx->rawCount.p,
+                       ctx->rawHits.p, ctx->counters.p, ctx->outStatus.p, ctx->outSeq.p);
+        }
+        const bool replay = graph && gc.exec && gc.key == key;
         HIP_TRY(hipEventRecord(ctx->ev[2], s));
         int bi = 0;
-        for (int done = 0; done < n; done += B, bi++) {
-            const int nb = std::min(B, n - done);
-            hipEvent_t* e = &ctx->batchEv[size_t(3 * bi)];
-            HIP_TRY(hipEventRecord(e[0], s));
-            launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->queryCap, ctx->queryRegMax,
-                         o.exclude_ambiguous,
-                         o.candidate_limit, ctx->candIds.p, ctx->candCount.p,
-                         reinterpret_cast<u32*>(ctx->metaA.p), ctx->rawCount.p, s);
-            launch_sum_counts(ctx->rawCount.p, ctx->candCount.p, nb, ctx->rawHits.p, s);
-            HIP_TRY(hipEventRecord(e[1], s));
-            launch_reset_counters(ctx->counters.p, nullptr, s);  // work queue of this batch
-            P.firstAnchor = first + done;
-            P.numAnchors = nb;
-            P.outStatus = ctx->outStatus.p + size_t(done);
-            P.outSeq = ctx->outSeq.p + size_t(done) * ctx->outPitch;
-            const int blocks = std::max(1, std::min(nb, ctx->numCUs * blocksPerCU));
-            launch_correct(P, blocks, T, ctx->msaChunk, s);
-            HIP_TRY(hipGetLastError());
-            HIP_TRY(hipEventRecord(e[2], s));
-            ctx->stats.launches++;
-            if (overlapCopies) {
-                hipStream_t cs = ctx->copyStream;
-                HIP_TRY(hipStreamWaitEvent(cs, e[2], 0));
-                HIP_TRY(hipMemcpyAsync(r->anchor_status + done, ctx->outStatus.p + done, size_t(nb),
-                                       hipMemcpyDeviceToHost, cs));
-                copy_rows_to_host(r->anchor_sequences + size_t(done) * size_t(r->sequence_pitch),
-                                  size_t(r->sequence_pitch), ctx->outSeq.p + size_t(done) * size_t(ctx->outPitch),
-                                  size_t(ctx->outPitch), size_t(ctx->rs.maxLen), size_t(nb), cs);
+        if (replay) {
+            bi = gc.launches;
+            ctx->stats.launches = gc.launches;
+        } else {
+            if (graph) {
+                gc.release();
+                HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            }
+            try {
+                launch_fill_u64(reinterpret_cast<u64*>(ctx->rawHits.p), 2, 0, s);  // totals over the call
+                launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);  // candidate results over the call
+                for (int done = 0; done < n; done += B, bi++) {
+                    const int nb = std::min(B, n - done);
+                    hipEvent_t* e = &ctx->batchEv[size_t(3 * bi)];
+                    if (!graph) HIP_TRY(hipEventRecord(e[0], s));
+                    launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->queryCap,
+                                 ctx->queryRegMax, o.exclude_ambiguous, o.candidate_limit, ctx->candIds.p,
+                                 ctx->candCount.p, reinterpret_cast<u32*>(ctx->metaA.p), ctx->rawCount.p, s);
+                    launch_sum_counts(ctx->rawCount.p, ctx->candCount.p, nb, ctx->rawHits.p, s);
+                    if (!graph) HIP_TRY(hipEventRecord(e[1], s));
+                    launch_reset_counters(ctx->counters.p, nullptr, s);  // work queue of this batch
+                    P.firstAnchor = first + done;
+                    P.numAnchors = nb;
+                    P.outStatus = ctx->outStatus.p + size_t(done);
+                    P.outSeq = ctx->outSeq.p + size_t(done) * ctx->outPitch;
+                    const int blocks = std::max(1, std::min(nb, ctx->numCUs * blocksPerCU));
+                    launch_correct(P, blocks, T, ctx->msaChunk, s);
+                    HIP_TRY(hipGetLastError());
+                    HIP_TRY(hipEventRecord(e[2], s));
+                    ctx->stats.launches++;
+                    if (overlapCopies) {
+                        hipStream_t cs = ctx->copyStream;
+                        HIP_TRY(hipStreamWaitEvent(cs, e[2], 0));
+                        HIP_TRY(hipMemcpyAsync(r->anchor_status + done, ctx->outStatus.p + done, size_t(nb),
+                                               hipMemcpyDeviceToHost, cs));
+                        copy_rows_to_host(r->anchor_sequences + size_t(done) * size_t(r->sequence_pitch),
+                                          size_t(r->sequence_pitch),
+                                          ctx->outSeq.p + size_t(done) * size_t(ctx->outPitch), size_t(ctx->outPitch),
+                                          size_t(ctx->rs.maxLen), size_t(nb), cs);
+                    }
+                }
+                if (overlapCopies) {
+                    HIP_TRY(hipEventRecord(ctx->copyDone, ctx->copyStream));
+                    HIP_TRY(hipStreamWaitEvent(s, ctx->copyDone, 0));
+                } else {
+                    HIP_TRY(hipMemcpyAsync(r->anchor_status, ctx->outStatus.p, size_t(n), hipMemcpyDeviceToHost, s));
+                    copy_rows_to_host(r->anchor_sequences, size_t(r->sequence_pitch), ctx->outSeq.p,
+                                      size_t(ctx->outPitch), size_t(ctx->rs.maxLen), size_t(n), s);
+                }
+                if (graph) {
+                    HIP_TRY(hipMemcpyAsync(ctx->hInts.p, ctx->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+                    HIP_TRY(hipMemcpyAsync(ctx->hU64.p, ctx->rawHits.p, 2 * sizeof(unsigned long long),
+                                           hipMemcpyDeviceToHost, s));
+                    HIP_TRY(hipStreamEndCapture(s, &gc.graph));
+                    HIP_TRY(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+                    gc.key = key;
+                    gc.launches = bi;
+                }
+            } catch (const HipError&) {
+                if (graph) {
+                    hipGraph_t g = nullptr;
+                    if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+                    gc.release();
+                }
+                throw;
             }
         }
-        if (overlapCopies) {
-            HIP_TRY(hipEventRecord(ctx->copyDone, ctx->copyStream));
-            HIP_TRY(hipStreamWaitEvent(s, ctx->copyDone, 0));
-        } else {
-            HIP_TRY(hipMemcpyAsync(r->anchor_status, ctx->outStatus.p, size_t(n), hipMemcpyDeviceToHost, s));
-            copy_rows_to_host(r->anchor_sequences, size_t(r->sequence_pitch), ctx->outSeq.p, size_t(ctx->outPitch),
-                              size_t(ctx->rs.maxLen), size_t(n), s);
-        }
+        if (graph) HIP_TRY(hipGraphLaunch(gc.exec, s));
         if (o.correct_candidates) {
-            HIP_TRY(hipMemcpyAsync(ctx->hInts.p, ctx->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            if (!graph) HIP_TRY(hipMemcpyAsync(ctx->hInts.p, ctx->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             const int nc = ctx->hInts.p[1];
             if (nc > candCap) return fail(CARE_ERR_CAPACITY, "candidate_capacity exceeded (%d results)", nc);
@@ -1217,11 +1293,13 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
             }
             r->num_candidates = nc;
         }
-        HIP_TRY(hipMemcpyAsync(ctx->hU64.p, ctx->rawHits.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        if (!graph)
+            HIP_TRY(hipMemcpyAsync(ctx->hU64.p, ctx->rawHits.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   s));
         HIP_TRY(hipEventRecord(ctx->ev[3], s));
         HIP_TRY(hipEventSynchronize(ctx->ev[3]));
         float qms = 0, cms = 0;
-        for (int i = 0; i < bi; i++) {
+        for (int i = 0; i < (graph ? 0 : bi); i++) {
             float t1 = 0, t2 = 0;
             HIP_TRY(hipEventElapsedTime(&t1, ctx->batchEv[size_t(3 * i)], ctx->batchEv[size_t(3 * i + 1)]));
             HIP_TRY(hipEventElapsedTime(&t2, ctx->batchEv[size_t(3 * i + 1)], ctx->batchEv[size_t(3 * i + 2)]));
@@ -1669,7 +1747,7 @@ int gate_columns_impl(care_context* ctx, uint64_t first, int n, float* features,
         res.candidate_sequences = candSeqs.data();
     }
     r.cands = cands;
-    const int rc = correct_batch_impl(ctx, first, n, &res);
+    const int rc = correct_batch_impl(ctx, first, n, &res, false);
     ctx->opt.correct_candidates = cc;
     ctx->opt.correction_type_cands = ctc;
     r.cands = false;

@@ -235,6 +235,18 @@ typedef struct care_batch_result {
  * candidatecorrection,correction}kernels.cu). */
 int care_correct_batch(care_context* ctx, uint64_t first, int n, care_batch_result* result);
 
+/* care_correct_batch through a HIP graph: the device work of the call (every
+ * internal batch's query, count, queue reset and correction kernels, the
+ * anchor result copies and the counter copies) is captured on the first
+ * call and replayed by later calls with the same range, result arrays,
+ * options, launch shapes and device buffers; any change captures anew.
+ * anchor_status and anchor_sequences must be page-locked
+ * (care_host_buffer_alloc).  Results equal care_correct_batch's; the batch
+ * statistics give the total device time only.  Replaces the stage-by-stage
+ * launches of the reference's correction pipeline with their host
+ * synchronisations between stages (include/gpu/gpucorrector.cuh:336,433,3105). */
+int care_correct_batch_graph(care_context* ctx, uint64_t first, int n, care_batch_result* result);
+
 /* care_correct_batch without any allocation: no counterpart in the
  * reference; for callers that cannot allocate device memory (code under
  * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Same

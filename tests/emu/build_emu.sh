@@ -21,6 +21,15 @@ if [ "$1" = "checked" ]; then
     HOSTFLAGS="$HOSTFLAGS -g $SAN"
 fi
 mkdir -p "$out"
+# up to date: both outputs newer than every source they are built from
+lib="$out/libcare_amd_emu.so"
+exe="$out/care-gpu-emu"
+if [ -f "$lib" ] && [ -f "$exe" ] && [ -z "$(find "$root/care_amd/csrc" "$root/care_amd/host" "$root/include" \
+        "$here/hip" "$here/build_emu.sh" -type f -newer "$lib" 2>/dev/null)" ] && \
+        [ -z "$(find "$root/care_amd/csrc" "$root/care_amd/host" "$root/include" "$here/hip" "$here/build_emu.sh" \
+        -type f -newer "$exe" 2>/dev/null)" ]; then
+    exit 0
+fi
 g++ $FLAGS -shared -x c++ -o "$out/libcare_amd_emu.so" \
     "$root/care_amd/csrc/api.hip" "$root/care_amd/csrc/minhash.hip" "$root/care_amd/csrc/correct.hip" \
     "$root/care_amd/csrc/mlp.hip" "$root/care_amd/csrc/store.hip"

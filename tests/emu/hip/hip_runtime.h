@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -332,8 +333,28 @@ inline int poison_byte() {
     return v;
 }
 
+// Stream capture: while a capture is open, launches and copies are recorded
+// (arguments by value) instead of run; launching the graph runs them in
+// order.
+inline std::vector<std::function<void()>>*& capture_list() {
+    static std::vector<std::function<void()>>* c = nullptr;
+    return c;
+}
+
+template <class K, class... Args>
+void launch_run(const char* name, K kernel, dim3 grid, dim3 block, size_t ldsBytes, Args... args);
+
 template <class K, class... Args>
 void launch_named(const char* name, K kernel, dim3 grid, dim3 block, size_t ldsBytes, Args... args) {
+    if (capture_list()) {
+        capture_list()->push_back([=]() { launch_run(name, kernel, grid, block, ldsBytes, args...); });
+        return;
+    }
+    launch_run(name, kernel, grid, block, ldsBytes, args...);
+}
+
+template <class K, class... Args>
+void launch_run(const char* name, K kernel, dim3 grid, dim3 block, size_t ldsBytes, Args... args) {
     if (block.x > block_cap()) block.x = block_cap();
     const unsigned nthreads = block.x;
     lds().assign(ldsBytes ? ldsBytes : 1, 0);
@@ -612,8 +633,12 @@ inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t* a, const void* 
 }
 inline hipError_t hipMemcpy2DAsync(void* d, size_t dpitch, const void* s, size_t spitch, size_t width,
                                    size_t height, hipMemcpyKind, hipStream_t = nullptr) {
-    for (size_t r = 0; r < height; r++)
-        std::memcpy(static_cast<char*>(d) + r * dpitch, static_cast<const char*>(s) + r * spitch, width);
+    auto run = [=]() {
+        for (size_t r = 0; r < height; r++)
+            std::memcpy(static_cast<char*>(d) + r * dpitch, static_cast<const char*>(s) + r * spitch, width);
+    };
+    if (emu::capture_list()) emu::capture_list()->push_back(run);
+    else run();
     return hipSuccess;
 }
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
@@ -621,7 +646,46 @@ inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
     return hipSuccess;
 }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) {
-    std::memcpy(d, s, n);
+    if (emu::capture_list()) emu::capture_list()->push_back([=]() { std::memcpy(d, s, n); });
+    else std::memcpy(d, s, n);
+    return hipSuccess;
+}
+// graphs: the recorded work of one capture
+struct emu_graph {
+    std::vector<std::function<void()>> ops;
+};
+typedef emu_graph* hipGraph_t;
+typedef emu_graph* hipGraphExec_t;
+typedef void* hipGraphNode_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
+                            hipStreamCaptureModeRelaxed = 2 };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    if (emu::capture_list()) return hipErrorInvalidValue;
+    emu::capture_list() = new std::vector<std::function<void()>>();
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    if (!emu::capture_list()) return hipErrorInvalidValue;
+    *g = new emu_graph{std::move(*emu::capture_list())};
+    delete emu::capture_list();
+    emu::capture_list() = nullptr;
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
+    *e = new emu_graph{g->ops};
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    if (emu::capture_list()) return hipErrorInvalidValue;
+    for (auto& op : e->ops) op();
+    return hipSuccess;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+    delete e;
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) {
+    delete g;
     return hipSuccess;
 }
 inline hipError_t hipGetLastError() { return hipSuccess; }

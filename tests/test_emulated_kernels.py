@@ -105,10 +105,11 @@ def checked_exe():
     return CHECKED_EXE
 
 
+@pytest.mark.slow
 @pytest.mark.parametrize("name", ["se_small", "se_mixed_small", "pe_small_q2"])
 def test_checked_kernels_match_oracle(checked_exe, name, tmp_path):
     case = dict(SMALL[name])
-    case["args"] = case["args"] + ["--batchsize", "128"]
+    case["args"] = case["args"] + ["--batchsize", "512"]
     want = _oracle(dict(SMALL[name]), str(tmp_path))
     env = {"EMU_CUS": "256", "ASAN_OPTIONS": "detect_leaks=0:detect_stack_use_after_return=0"}
     assert _run(checked_exe, case, str(tmp_path), env) == want

@@ -282,3 +282,27 @@ def test_gpu_candidate_gate_columns_match_oracle(tmp_path):
     got, want = _cand_records(None, CAND_GPU, str(tmp_path))
     assert len(want) > 100
     _assert_same_records(got, want)
+
+
+# What the trained network gains over the method without it, on a held-out
+# read set of the generator it was trained on (a seed used neither for
+# training nor for the threshold): the same reduced pipeline (top 20
+# candidates, no refinement) deciding by the classic thresholds loses true
+# positives, accepting every consensus multiplies false positives; the gate
+# keeps the plain path's true positives within 2 % at far fewer false
+# positives than accepting everything.
+GAIN_CASE = {"profile": "repeat-pe", "seed": 20240777, "genome": 2000000}
+
+
+@pytest.mark.gpu
+def test_gpu_gate_gain_over_reduced_pipeline_without_it(tmp_path):
+    from care_amd import gate as gate_mod
+    print("gate ablation on a held-out repeat-pe set (seed %d)" % GAIN_CASE["seed"], flush=True)
+    reads, truth = helpers.simulate(GAIN_CASE, str(tmp_path))
+    res = gate_mod.ablation(reads, truth, str(tmp_path))
+    print(res, flush=True)
+    plain, classic, gated, accept = (res[k] for k in ("plain", "reduced_classic", "reduced_mlp_gate",
+                                                      "reduced_accept_all"))
+    assert gated["TP"] >= 1.1 * classic["TP"]
+    assert gated["FP"] * 4 <= accept["FP"]
+    assert gated["TP"] >= 0.98 * plain["TP"]
