@@ -214,11 +214,27 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=Tru
         query_ms.append(st.query_ms)
         log("bench: %s step %d %.1f ms (query %.1f ms, correct kernel %.1f ms)" % (
             "gated" if gate is not None else "plain", i + 1, step_ms[-1], st.query_ms, st.correct_ms))
+    # the same step as a HIP graph (care_correct_batch_graph): captured by the
+    # first call, replayed by the timed ones; results checked against the
+    # last eager step
+    graph_ms = None
+    if chunk == n:
+        ref_status = out[0][:n].copy()
+        eng.correct(0, n, candidate_capacity=cap, out=out, graph=True)
+        gms = []
+        for i in range(steps):
+            ts = time.perf_counter()
+            eng.correct(0, n, candidate_capacity=cap, out=out, graph=True)
+            gms.append((time.perf_counter() - ts) * 1e3)
+        if not np.array_equal(out[0][:n], ref_status):
+            raise RuntimeError("graph replay differs from the eager step")
+        graph_ms = sum(gms) / len(gms)
+        log("bench: %s graph step %.1f ms" % ("gated" if gate is not None else "plain", graph_ms))
     eng.close()
     return {"ms": sum(step_ms) / len(step_ms), "correct_ms": sum(corr_ms) / len(corr_ms),
             "query_ms": sum(query_ms) / len(query_ms), "launches": max(1, st.launches), "candidates": int(st.candidates),
             "raw_hits": int(st.raw_hits), "h": int(opt.num_hash_maps), "tuning": tuning, "build_ms": build_ms,
-            "corrected": int(st.corrected)}
+            "corrected": int(st.corrected), "graph_ms": graph_ms}
 
 
 def step_times(stdout):
@@ -414,7 +430,8 @@ def main():
         return {"reads_per_s": round(n / (r["ms"] / 1e3), 1), "ms_per_step": round(r["ms"], 3),
                 "query_kernel_ms": round(r["query_ms"], 3), "correct_kernel_ms": round(r["correct_ms"], 3),
                 "candidates_per_step": r["candidates"], "corrected_anchors": r["corrected"], "launch_shapes": r["tuning"],
-                "table_build_ms": round(r["build_ms"], 1)}
+                "table_build_ms": round(r["build_ms"], 1),
+                "graph_ms_per_step": None if r["graph_ms"] is None else round(r["graph_ms"], 3)}
 
     config = {"workload": w["label"], "reads": n, "plain": summary(plain)}
     if gated is not None:

@@ -450,28 +450,32 @@ struct ColumnStats {
     int minCov, maxCov;
 };
 
-// MSA properties over columns [first, last) (src/msa.cpp:304-331); one
-// thread, support summed in column order
-__device__ ColumnStats column_stats(const Lds& L, int first, int last) {
-    ColumnStats cs;
-    float mn = L.support[first];
-    float sum = 0.0f;
-    int mnc = L.coverage[first], mxc = L.coverage[first];
-    // one thread, in column order (the support sum is sequential FP32 as on
-    // the CPU); unrolled so that the LDS loads run ahead of the add chain
-#pragma unroll 8
-    for (int c = first; c < last; c++) {
-        const float s = L.support[c];
-        mn = s < mn ? s : mn;
-        sum += s;
+// MSA properties over columns [first, last) (src/msa.cpp:304-331) by one
+// wavefront (all lanes call): minima and maxima over the lanes (support is
+// non-negative, so its bit pattern orders like the value), the support sum
+// added in column order by lane 0 alone (sequential FP32, as on the CPU);
+// avgSupport is valid in lane 0, the other fields in every lane
+__device__ ColumnStats column_stats_wave(const Lds& L, int first, int last) {
+    const int lane = lane_id();
+    u32 mnBits = __float_as_uint(L.support[first]);
+    int mnc = L.coverage[first], mxc = mnc;
+    for (int c = first + lane; c < last; c += kWave) {
+        const u32 s = __float_as_uint(L.support[c]);
+        mnBits = s < mnBits ? s : mnBits;
         const int cv = L.coverage[c];
         mnc = cv < mnc ? cv : mnc;
         mxc = cv > mxc ? cv : mxc;
     }
+    ColumnStats cs;
+    cs.minSupport = __uint_as_float(wave_min_u32(mnBits));
+    cs.minCov = wave_min_int(mnc);
+    cs.maxCov = wave_max_int(mxc);
+    float sum = 0.0f;
+    if (lane == 0) {
+#pragma unroll 8
+        for (int c = first; c < last; c++) sum += L.support[c];
+    }
     cs.avgSupport = sum / float(last - first);
-    cs.minSupport = mn;
-    cs.minCov = mnc;
-    cs.maxCov = mxc;
     return cs;
 }
 
@@ -1000,8 +1004,10 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         if (m > 0 || forestA) {
             __shared__ float sMinSupport, sAvgSupport;
             __shared__ int sMinCov, sMaxCov, sHQ, sByCons;
+            ColumnStats cs0{};
+            if (wave == 0) cs0 = column_stats_wave(L, msa.ab, msa.ab + la);
             if (tid == 0) {
-                const ColumnStats cs = column_stats(L, msa.ab, msa.ab + la);
+                const ColumnStats cs = cs0;
                 const float mn = cs.minSupport, avg = cs.avgSupport;
                 const int mc = cs.minCov;
                 const bool allGood = fgeq(avg, P.avgSupportThreshold) && fgeq(mn, P.minSupportThreshold) &&
@@ -1148,11 +1154,9 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     if (lane == 0) P.candOutIds[slot] = cm.id;
                     ColumnStats ccs{};
                     if (forestC || gateC) {
-                        if (lane == 0) ccs = column_stats(L, qb, qe);
-                        ccs.avgSupport = __shfl(ccs.avgSupport, 0, kWave);
-                        ccs.minSupport = __shfl(ccs.minSupport, 0, kWave);
-                        ccs.minCov = __shfl(ccs.minCov, 0, kWave);
-                        ccs.maxCov = __shfl(ccs.maxCov, 0, kWave);
+                        ccs = column_stats_wave(L, qb, qe);
+                        ccs.avgSupport = __uint_as_float(
+                            u32(__builtin_amdgcn_readlane(int(__float_as_uint(ccs.avgSupport)), 0)));
                     }
                     // jaccard of the anchor and candidate column ranges
                     const float jac = float(min(ae, qe) - max(msa.ab, qb)) / float(max(ae, qe) - min(msa.ab, qb));

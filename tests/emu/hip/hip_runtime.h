@@ -490,6 +490,11 @@ inline float __uint_as_float(unsigned x) {
     std::memcpy(&f, &x, 4);
     return f;
 }
+inline unsigned __float_as_uint(float x) {
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    return u;
+}
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
