@@ -867,16 +867,32 @@ struct ResultView {
     std::vector<u8> flags;
     std::vector<u32> dataLen;
     std::vector<const char*> ptr;
-    void append(ResultStore& s) {
-        const char* base = s.data.finish();
-        readId.insert(readId.end(), s.readId.begin(), s.readId.end());
-        flags.insert(flags.end(), s.flags.begin(), s.flags.end());
-        dataLen.insert(dataLen.end(), s.dataLen.begin(), s.dataLen.end());
-        for (u64 off : s.dataOff) ptr.push_back(base + off);
-        std::vector<u32>().swap(s.readId);
-        std::vector<u8>().swap(s.flags);
-        std::vector<u32>().swap(s.dataLen);
-        std::vector<u64>().swap(s.dataOff);
+    // the entries of every store in store order, one thread per store
+    void collect(std::vector<ResultStore>& parts) {
+        std::vector<size_t> at(parts.size() + 1, readId.size());
+        for (size_t i = 0; i < parts.size(); i++) at[i + 1] = at[i] + parts[i].readId.size();
+        readId.resize(at.back());
+        flags.resize(at.back());
+        dataLen.resize(at.back());
+        ptr.resize(at.back());
+        std::vector<const char*> bases(parts.size());
+        for (size_t i = 0; i < parts.size(); i++) bases[i] = parts[i].data.finish();  // may map a file
+        std::vector<std::thread> pool;
+        for (size_t i = 0; i < parts.size(); i++) {
+            pool.emplace_back([&, i]() {
+                ResultStore& s = parts[i];
+                const size_t o = at[i];
+                std::copy(s.readId.begin(), s.readId.end(), readId.begin() + long(o));
+                std::copy(s.flags.begin(), s.flags.end(), flags.begin() + long(o));
+                std::copy(s.dataLen.begin(), s.dataLen.end(), dataLen.begin() + long(o));
+                for (size_t e = 0; e < s.dataOff.size(); e++) ptr[o + e] = bases[i] + s.dataOff[e];
+                std::vector<u32>().swap(s.readId);
+                std::vector<u8>().swap(s.flags);
+                std::vector<u32>().swap(s.dataLen);
+                std::vector<u64>().swap(s.dataOff);
+            });
+        }
+        for (auto& t : pool) t.join();
     }
 };
 
@@ -1075,6 +1091,9 @@ int runMain(int argc, char** argv) {
     // DMA), two sets: the device fills one while host threads turn the
     // other's corrections into edit lists.
     const int pitch = (rs->maxLen + 15) / 16 * 16;
+    // chunks of 2^17 anchors (larger ones measured slower: each call's
+    // result copy cannot overlap the next call), the last one split so that
+    // the host's work on the final results is short
     const int chunk = int(std::min<u64>(u64(1) << 17, std::max<u64>(numAnchors + (numAnchors & 1), 2)));
     const int candCap = o.eng.correct_candidates ? chunk * o.candResultsPerAnchor : 0;
     struct Slot {
@@ -1148,6 +1167,8 @@ int runMain(int argc, char** argv) {
     // halves.
     std::mutex mu;
     std::condition_variable cv;
+    double deviceBusy = 0, hostBusy = 0, firstResult = -1;  // seconds, for the STEP 2 breakdown
+    int deviceCalls = 0;
     int filled[2] = {0, 0};  // 1: results waiting for the host
     bool deviceDone = false;
     std::string deviceError;
@@ -1155,11 +1176,16 @@ int runMain(int argc, char** argv) {
     std::thread device([&]() {
         // pending anchor ranges as a stack, the next range on top
         std::vector<std::pair<u64, int>> work;
-        for (u64 first = 0; first < numAnchors; first += u64(chunk)) {
-            int n = int(std::min<u64>(u64(chunk), numAnchors - first));
+        const u64 tail = std::max<u64>(u64(chunk) / 8, 2);
+        for (u64 first = 0; first < numAnchors;) {
+            const u64 rem = numAnchors - first;
+            u64 take = std::min<u64>(u64(chunk), rem);
+            if (rem > tail && rem <= u64(chunk) + tail) take = rem - tail;  // a short final chunk
+            int n = int(take);
             if (o.eng.paired_end && (n & 1)) n++;  // whole pairs
             if (first + u64(n) > rs->n) n = int(rs->n - first);
             work.emplace_back(first, n);
+            first += u64(n);
         }
         std::reverse(work.begin(), work.end());
         int next = 0;
@@ -1178,7 +1204,11 @@ int runMain(int argc, char** argv) {
             r.candidate_read_ids = sl.candIds.data();
             r.candidate_sequences = sl.candSeqs.data();
             r.candidate_capacity = candCap;
+            const double tc = now_s();
             const int rc = care_correct_batch(ctx, first, n, &r);
+            deviceBusy += now_s() - tc;
+            deviceCalls++;
+            if (firstResult < 0) firstResult = now_s() - t1;
             if (rc == CARE_ERR_CAPACITY && n > 2) {
                 // candidate results beyond the slot: the two halves separately
                 std::printf("Candidate corrections of reads %llu..%llu exceed the result buffer: corrected in halves\n",
@@ -1214,26 +1244,28 @@ int runMain(int argc, char** argv) {
             if (order.size() <= k) break;
             slot = order[k];
         }
+        const double th = now_s();
         processSlot(*slots[size_t(slot)]);
+        hostBusy += now_s() - th;
         std::lock_guard<std::mutex> lk(mu);
         filled[slot] = 0;
         cv.notify_all();
     }
     device.join();
     if (!deviceError.empty()) throw std::runtime_error("engine: " + deviceError);
+    const double tLoop = now_s();
     ResultView anchors, cands;
+    anchors.collect(anchorParts);
+    cands.collect(candParts);
     u64 spilled = 0;
-    for (auto& p : anchorParts) {
-        anchors.append(p);
-        spilled += p.data.spilledBytes();
-    }
-    for (auto& p : candParts) {
-        cands.append(p);
-        spilled += p.data.spilledBytes();
-    }
+    for (auto& p : anchorParts) spilled += p.data.spilledBytes();
+    for (auto& p : candParts) spilled += p.data.spilledBytes();
     if (spilled) std::printf("Correction results: %llu bytes kept in temporary files in %s\n",
                              (unsigned long long)spilled, o.tempdir.c_str());
     const double t2 = now_s();
+    std::printf("Correction: %d device calls busy %.3fs (first results after %.3fs), host edit lists %.3fs, "
+                "result collection %.3fs\n",
+                deviceCalls, deviceBusy, firstResult, hostBusy, t2 - tLoop);
     std::printf("# elapsed time (STEP2): %.6fs\n", t2 - t1);
     std::printf("Correction throughput : ~%.1f reads/second.\n", double(numAnchors) / (t2 - t1));
 
