@@ -166,3 +166,27 @@ def test_candidate_result_overflow_recorrects_in_halves(emu_exe, tmp_path):
     assert p.returncode == 0, p.stdout[-3000:]
     assert "corrected in halves" in p.stdout
     assert helpers.md5(os.path.join(out, "r.fastq")) == want
+
+
+def test_emulated_two_gzip_mate_files_and_gzip_output(emu_exe, tmp_path):
+    """The mates of a paired-end case as two gzip'ed files, gzip output:
+    the decompressed output equals the interleaved single-file run's."""
+    import gzip
+    case = SMALL["pe_small_q2"]
+    d = str(tmp_path)
+    want = _run(emu_exe, case, d)
+    reads, _ = helpers.simulate(case, d)
+    with open(reads) as f:
+        lines = f.readlines()
+    m1, m2 = os.path.join(d, "m1.fq.gz"), os.path.join(d, "m2.fq.gz")
+    with gzip.open(m1, "wt") as a, gzip.open(m2, "wt") as b:
+        for r in range(len(lines) // 4):
+            (a if r % 2 == 0 else b).writelines(lines[4 * r:4 * r + 4])
+    out = os.path.join(d, "out2")
+    p = subprocess.run([emu_exe, "-i", m1, "-i", m2, "-o", "r.fq.gz", "-d", out, "-g", "0", "-t", "4", "--gzoutput"]
+                       + _args(case, d), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert p.returncode == 0, p.stdout[-3000:]
+    plain = os.path.join(d, "r2.fastq")
+    with gzip.open(os.path.join(out, "r.fq.gz"), "rb") as src, open(plain, "wb") as dst:
+        dst.write(src.read())
+    assert helpers.md5(plain) == want

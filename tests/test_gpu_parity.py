@@ -131,3 +131,43 @@ def test_reduced_pipeline_matches_oracle(tmp_path):
                          + case["args"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
     assert orc.returncode == 0, orc.stdout[-2000:]
     assert helpers.md5(str(tmp_path / "g" / "g.fastq")) == helpers.md5(str(tmp_path / "o" / "o.fastq"))
+
+
+def _split_gz_mates(reads, d):
+    """The interleaved mates of `reads` as two gzip'ed files."""
+    import gzip
+    with open(reads) as f:
+        lines = f.readlines()
+    m1, m2 = os.path.join(d, "m1.fastq.gz"), os.path.join(d, "m2.fastq.gz")
+    with gzip.open(m1, "wt") as a, gzip.open(m2, "wt") as b:
+        for r in range(0, len(lines) // 4):
+            (a if r % 2 == 0 else b).writelines(lines[4 * r:4 * r + 4])
+    return m1, m2
+
+
+def _two_file_gz_run(exe, tmp_path):
+    """pe_quality2 as two gzip'ed mate files with --gzoutput: the decompressed
+    output equals the golden digest of the interleaved single-file run."""
+    import gzip
+    import subprocess
+    case = CASES["pe_quality2"]
+    d = str(tmp_path)
+    reads, _ = helpers.simulate(case, d)
+    g = helpers.golden()["pe_quality2"]
+    assert helpers.md5(reads) == g["input_md5"]
+    m1, m2 = _split_gz_mates(reads, d)
+    out = tmp_path / "out"
+    p = subprocess.run([exe, "-i", m1, "-i", m2, "-o", "gpu.fastq.gz", "-d", str(out), "-g", "0", "-t", "8",
+                        "--gzoutput"] + case_args(case, d), stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=600)
+    _say(p.stdout[-1500:])
+    assert p.returncode == 0
+    plain = str(tmp_path / "gpu.fastq")
+    with gzip.open(str(out / "gpu.fastq.gz"), "rb") as src, open(plain, "wb") as dst:
+        dst.write(src.read())
+    assert helpers.md5(plain) == g["output_md5"]
+
+
+def test_gpu_two_gzip_mate_files_and_gzip_output(tmp_path):
+    _say("pe_quality2 from two gzip'ed mate files, gzip output")
+    _two_file_gz_run(care_amd.CARE_GPU, tmp_path)
