@@ -170,6 +170,10 @@ ENGINE_SYMBOLS = {
                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p, ctypes.c_uint64,
                                                     ctypes.POINTER(ctypes.c_uint64)]),
+    "care_gate_columns_plain": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_uint64,
+                                               ctypes.POINTER(ctypes.c_uint64)]),
     "care_mlp_load_candidates": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "care_readstore_gather_sequences_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
@@ -574,11 +578,13 @@ class Engine:
         assert p.shape == (MLP_PARAMS,)
         self._ck(self.lib.care_mlp_load_candidates(self.ctx, p.ctypes.data))
 
-    def gate_columns(self, first, n, capacity=None, candidates=False):
+    def gate_columns(self, first, n, capacity=None, candidates=False, plain=False):
         """Decision columns of the gated mode: features [c][32], read ids,
         positions, original and consensus bases (candidates: those of the
-        candidate corrections, forward-strand positions and bases)."""
-        fn = self.lib.care_gate_columns_candidates if candidates else self.lib.care_gate_columns
+        candidate corrections, forward-strand positions and bases; plain:
+        over the plain method's MSAs, every candidate and refinement)."""
+        fn = (self.lib.care_gate_columns_candidates if candidates else
+              self.lib.care_gate_columns_plain if plain else self.lib.care_gate_columns)
         cap = capacity if capacity is not None else max(1, n) * 8
         while True:
             f = np.zeros((cap, MLP_FEATURES), dtype=np.float32)

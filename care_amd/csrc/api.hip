@@ -213,6 +213,7 @@ struct care_context {
         int cap = 0;
         bool on = false;
         bool cands = false;  // record candidate decision columns instead of the anchors'
+        bool refine = false; // the plain method's columns: all candidates, refinement first
     } record;
 };
 
@@ -1086,6 +1087,7 @@ int prepare_params(care_context* ctx, CorrectParams& P) {
         const float t = std::min(std::max(o.threshold_anchor, 1e-6f), 1.0f - 1e-6f);
         P.gateLogit = std::log(t / (1.0f - t));
         if (ctx->record.on) {
+            P.gatedRefine = ctx->record.refine ? 1 : 0;
             P.gateFeat = ctx->record.feat.p;
             P.gateIds = ctx->record.ids.p;
             P.gatePos = ctx->record.pos.p;
@@ -1149,7 +1151,8 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
         if (lds > size_t(kMaxLdsBytes))
             return fail(CARE_ERR_CAPACITY, "reads of length %d need %zu bytes of LDS per correction workgroup (limit %d)",
                         ctx->rs.maxLen, lds, kMaxLdsBytes);
-        const int blocksPerCU = ctx->blocksPerCU > 0 ? ctx->blocksPerCU : correct_resident_blocks(T, lds, gatedK);
+        const int blocksPerCU = ctx->blocksPerCU > 0 ? ctx->blocksPerCU
+                                                     : correct_resident_blocks(T, lds, gatedK | (P.gatedRefine ? 4 : 0));
         hipStream_t s = ctx->stream;
         P.candIds = ctx->candIds.p;
         P.candCount = ctx->candCount.p;
@@ -1680,7 +1683,7 @@ int care_mlp_infer(care_context* ctx, const float* features, uint64_t n, const f
 namespace {
 int gate_columns_impl(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
                       uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count,
-                      bool cands);
+                      bool cands, bool plain = false);
 }
 
 int care_gate_columns(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
@@ -1694,12 +1697,17 @@ int care_gate_columns_candidates(care_context* ctx, uint64_t first, int n, float
     return gate_columns_impl(ctx, first, n, features, read_ids, positions, orig, cons, capacity, count, true);
 }
 
+int care_gate_columns_plain(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
+                            uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count) {
+    return gate_columns_impl(ctx, first, n, features, read_ids, positions, orig, cons, capacity, count, false, true);
+}
+
 }  // extern "C"
 
 namespace {
 int gate_columns_impl(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
                       uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count,
-                      bool cands) {
+                      bool cands, bool plain) {
     if (!ctx || !features || !read_ids || !positions || !orig || !cons || !count)
         return fail(CARE_ERR_ARGUMENT, "null argument");
     if (ctx->opt.correction_type != 3) return fail(CARE_ERR_STATE, "gate columns need correction_type 3");
@@ -1747,7 +1755,13 @@ int gate_columns_impl(care_context* ctx, uint64_t first, int n, float* features,
         res.candidate_sequences = candSeqs.data();
     }
     r.cands = cands;
+    // the plain method's columns: every candidate, refinement before the decision
+    const int limit = ctx->opt.candidate_limit;
+    if (plain) ctx->opt.candidate_limit = 0;
+    r.refine = plain;
     const int rc = correct_batch_impl(ctx, first, n, &res, false);
+    ctx->opt.candidate_limit = limit;
+    r.refine = false;
     ctx->opt.correct_candidates = cc;
     ctx->opt.correction_type_cands = ctc;
     r.cands = false;

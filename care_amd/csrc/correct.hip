@@ -653,9 +653,11 @@ __device__ void gate_candidate(const CorrectParams& P, const Lds& L, const DevRe
 }  // namespace
 
 // kGated: the learned-gate mode (correctionType 3), kGateCands: with the
-// gate on candidate columns too (correctionTypeCands 3); compiled separately
-// so that their register use does not lower the occupancy of the other modes.
-template <bool kGated, bool kGateCands>
+// gate on candidate columns too (correctionTypeCands 3), kRefine: refinement
+// rounds (the plain modes; in the gated mode only to record the decision
+// columns of the plain method); compiled separately so that their register
+// use does not lower the occupancy of the other modes.
+template <bool kGated, bool kGateCands, bool kRefine = !kGated>
 __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) {
     HIP_DYNAMIC_SHARED(uint4, ldsRaw)  // uint4: 16-byte aligned dynamic LDS
     const Lds L = carve(reinterpret_cast<char*>(ldsRaw), dims);
@@ -917,7 +919,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         // new_columns past it, gate features two columns past a corrected read
         const int margin = P.correctCandidates ? P.newColumns + 2 : 2;
         Msa msa = build_msa(P, L, dims, cur, m, la, margin, &sMin, &sMax);
-        const int maxIter = kGated ? 0 : P.maxRefinements;  // the gated mode skips refinement
+        const int maxIter = kRefine ? P.maxRefinements : 0;  // the gated mode skips refinement
         for (int iter = 0; iter < maxIter && m > 0; iter++) {
             // src/msa.cpp:532-734
             if (tid == 0) {
@@ -1236,6 +1238,8 @@ int correct_resident_blocks(int threads, size_t ldsBytes, int gated) {
     int nb = 0;
     const hipError_t e =
         (gated & 2)   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<true, true>, threads, ldsBytes)
+        : (gated & 4) ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<true, false, true>, threads,
+                                                                     ldsBytes)
         : (gated & 1) ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<true, false>, threads, ldsBytes)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_correct<false, false>, threads, ldsBytes);
     return e == hipSuccess && nb > 0 ? nb : 1;
@@ -1252,6 +1256,8 @@ void launch_correct(const CorrectParams& p, int blocks, int threads, int chunk, 
     // the occupancy of the anchor-only gated mode
     if (gm & 2)
         hipLaunchKernelGGL((k_correct<true, true>), dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
+    else if ((gm & 1) && p.gatedRefine)
+        hipLaunchKernelGGL((k_correct<true, false, true>), dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
     else if (gm & 1)
         hipLaunchKernelGGL((k_correct<true, false>), dim3(blocks), dim3(threads), d.bytes(), stream, p, d);
     else

@@ -134,6 +134,7 @@ struct CorrectParams {
     DevForest forestC;
     float estimatedCoverage;
     int maxRefinements;         // different-region refinement rounds (the reference: 5)
+    int gatedRefine;            // correctionType 3 recording the plain method's columns: refine first
     // learned gate (correctionType 3): BF16 weight matrices and the FP32
     // parameter vector of the gate, logit threshold of acceptance
     const unsigned short* gateW1;

@@ -91,13 +91,15 @@ def options(**kw):
     return care_amd.default_options(**o)
 
 
-def collect(reads, truth, device=0):
-    """Gate decision columns of one read set and their labels."""
+def collect(reads, truth, device=0, plain=False):
+    """Gate decision columns of one read set and their labels: over the
+    reduced pipeline's MSAs (the ones the gate decides on), or with plain
+    over the plain method's (every candidate, refinement)."""
     rs = care_amd.ReadSet([reads], paired_end=True, qual_bits=QUAL_BITS)
     with care_amd.Engine(options(), device=device) as eng:
         eng.upload_readset(rs)
         eng.build()
-        f, ids, pos, orig, cons = eng.gate_columns(0, rs.n)
+        f, ids, pos, orig, cons = eng.gate_columns(0, rs.n, plain=plain)
     t = truth_codes(truth, rs.n, rs.max_length)
     tb = t[ids, pos]
     keep = tb < 4
@@ -127,7 +129,7 @@ def train(args):
         sets = simulate_many(seeds[w0:w0 + args.workers], work, workers=args.workers)
         print("gate: simulated %d read sets in %.1f s" % (len(sets), time.perf_counter() - t0), flush=True)
         for reads, truth in sets:
-            x, y = collect(reads, truth)
+            x, y = collect(reads, truth, plain=args.columns == "plain")
             xs.append(x)
             ys.append(y)
             os.remove(reads)
@@ -145,7 +147,8 @@ def train(args):
     acc = float(np.mean((z >= 0) == (y == 1)))
     steps = args.epochs * ((len(y) + args.batch - 1) // args.batch)
     flops = 3 * 9248.0 * len(y) * args.epochs
-    info = {"columns": int(len(y)), "positive": float(y.mean()), "epochs": args.epochs, "batch": args.batch,
+    info = {"columns": int(len(y)), "column_source": args.columns, "positive": float(y.mean()),
+            "epochs": args.epochs, "batch": args.batch,
             "lr": args.lr, "epoch_loss": [float(v) for v in losses], "train_accuracy": acc,
             "train_seconds": t2 - t1, "adam_steps": steps, "train_tflops": flops / (t2 - t1) / 1e12}
     print("gate: " + json.dumps(info), flush=True)
@@ -474,6 +477,8 @@ def main(argv=None):
     t.add_argument("--workers", type=int, default=8)
     t.add_argument("--work", default=None)
     t.add_argument("--out", default=DEFAULT_GATE)
+    t.add_argument("--columns", choices=["reduced", "plain"], default="reduced",
+                   help="training columns from the reduced pipeline the gate runs in, or from the plain method")
     tm = sub.add_parser("timing")
     tm.add_argument("--rows", type=int, default=4_000_000)
     tm.add_argument("--epochs", type=int, default=1)

@@ -368,6 +368,14 @@ int care_gate_columns_candidates(care_context* ctx, uint64_t first, int n, float
                                  uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity,
                                  uint64_t* count);
 
+/* care_gate_columns over the MSAs of the plain method: every candidate (the
+ * context's candidate_limit is not applied) and max_refinements rounds of
+ * refinement before the decision columns are taken, as the plain path builds
+ * them; the training columns of a gate learned from the plain method.  Needs
+ * correction_type 3; no counterpart in the reference. */
+int care_gate_columns_plain(care_context* ctx, uint64_t first, int n, float* features, uint32_t* read_ids,
+                            uint16_t* positions, uint8_t* orig, uint8_t* cons, uint64_t capacity, uint64_t* count);
+
 /* Timing and work counters of the last care_correct_batch call. */
 typedef struct care_batch_stats {
     double query_ms;       /* minhash query kernel */

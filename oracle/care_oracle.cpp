@@ -66,6 +66,7 @@ struct Options {
     int correctionType = 0;      // 0 classic, 1 forest, 3 learned gate (engine extension)
     int candidateLimit = 0;      // top candidates by map hits (0 = all, the reference)
     int maxRefinements = 5;      // different-region refinement rounds (the reference: 5)
+    bool gateRefine = false;     // correctionType 3 with refinement: the plain method's decision columns
     int gateConstant = 1;        // correctionType 3: the oracle's gate accepts (1) or rejects (-1) every column
     std::string gateDump;        // correctionType 3: binary record of every decision column
     int gateConstantCands = 1;   // correctionTypeCands 3: the same for candidate columns
@@ -1209,7 +1210,7 @@ struct Corrector {
         buildMsa();
 
         // refinement (include/corrector.hpp:1307-1421); none in the gated mode
-        for (int it = 0; it < (opt.correctionType == 3 ? 0 : opt.maxRefinements); it++) {
+        for (int it = 0; it < (opt.correctionType == 3 && !opt.gateRefine ? 0 : opt.maxRefinements); it++) {
             std::vector<char> remove;
             if (!msa.differentRegion(int(opt.estimatedCoverage), remove)) break;
             std::vector<Cand> kept;
@@ -1562,6 +1563,7 @@ Options parseArgs(int argc, char** argv) {
         else if (a == "--correctionTypeCands") o.correctionTypeCands = std::stoi(next());
         else if (a == "--candidateLimit") o.candidateLimit = std::stoi(next());
         else if (a == "--refinements") o.maxRefinements = std::stoi(next());
+        else if (a == "--gateRefine") o.gateRefine = true;
         else if (a == "--gateConstant") o.gateConstant = std::stoi(next());
         else if (a == "--gateDump") o.gateDump = next();
         else if (a == "--gateConstantCands") o.gateConstantCands = std::stoi(next());

@@ -194,6 +194,37 @@ def test_emulated_gate_columns_match_oracle(emu, name, tmp_path):
     assert np.array_equal(got, want)
 
 
+def _plain_columns(lib, case, d):
+    """Decision columns over the plain method's MSAs (every candidate,
+    refinement first): the engine's care_gate_columns_plain and the oracle's
+    dump of the same mode."""
+    reads, _ = helpers.simulate(case, d)
+    dump = os.path.join(d, "gate_plain.bin")
+    p = subprocess.run([helpers.ORACLE_EXE, "-i", reads, "-o", "r.fastq", "-d", os.path.join(d, "oracle_plain"), "-t",
+                        "4", "--correctionType", "3", "--candidateLimit", "0", "--gateRefine", "--gateConstant", "1",
+                        "--gateDump", dump] + case["args"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert p.returncode == 0, p.stdout[-2000:]
+    want = np.sort(np.fromfile(dump, dtype=REC), order=["id", "pos"])
+    rs = _readset(case, d)
+    eng = care_amd.Engine(care_amd.default_options(**_options(case)), device=0, lib=lib)
+    eng.upload_readset(rs)
+    eng.build()
+    f, ids, pos, orig, cons = eng.gate_columns(0, rs.n, plain=True)
+    reduced = _records(eng, rs)
+    eng.close()
+    got = np.zeros(len(ids), dtype=REC)
+    got["id"], got["pos"], got["orig"], got["cons"], got["f"] = ids, pos, orig, cons, f
+    return np.sort(got, order=["id", "pos"]), want, reduced
+
+
+def test_emulated_plain_method_columns_match_oracle(emu, tmp_path):
+    got, want, reduced = _plain_columns(emu, SMALL["pe_q"], str(tmp_path))
+    assert len(want) > 20
+    _assert_same_records(got, want)
+    # every candidate and refinement: other MSAs than the reduced pipeline's
+    assert len(got) != len(reduced) or not np.array_equal(got, reduced)
+
+
 def _assert_same_records(got, want):
     assert len(got) == len(want), (len(got), len(want))
     bad = np.nonzero(got != want)[0]
@@ -226,6 +257,14 @@ def test_emulated_candidate_gate_columns_match_oracle(emu, tmp_path):
 def test_gpu_gated_mode_matches_oracle(gate, tmp_path):
     print("gated mode, constant gate %d" % gate, flush=True)
     assert _engine_run(care_amd.CARE_GPU, GPU_CASE, str(tmp_path), gate) == _oracle(GPU_CASE, str(tmp_path), gate)
+
+
+@pytest.mark.gpu
+def test_gpu_plain_method_columns_match_oracle(tmp_path):
+    print("plain-method gate columns", flush=True)
+    got, want, _ = _plain_columns(None, GPU_CASE, str(tmp_path))
+    assert len(want) > 100
+    _assert_same_records(got, want)
 
 
 @pytest.mark.gpu
