@@ -107,9 +107,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def prepare(workload, cache):
+def prepare(workload, cache, genome=None):
     import care_amd
-    w = WORKLOADS[workload]
+    w = dict(WORKLOADS[workload])
+    if genome:
+        w["genome"] = genome
     os.makedirs(cache, exist_ok=True)
     reads = os.path.join(cache, "%s_%d_%d.fastq" % (w["profile"], w["seed"], w["genome"]))
     truth = reads + ".truth" if w.get("truth", True) else os.devnull
@@ -303,6 +305,8 @@ def main():
     ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--genome", type=int, default=0,
+                    help="genome size of the workload's generator (default: the workload's own)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -313,8 +317,12 @@ def main():
 
     import care_amd
     from care_amd import gate as gate_mod
-    w = WORKLOADS[args.workload]
-    reads, truth = prepare(args.workload, args.cache)
+    w = dict(WORKLOADS[args.workload])
+    if args.genome:
+        w["genome"] = args.genome
+        w["label"] = re.sub(r" \([0-9.]+[kM] reads\)", "",
+                            re.sub(r"\d+ Mbp genome", "%g Mbp genome" % (args.genome / 1e6), w["label"]))
+    reads, truth = prepare(args.workload, args.cache, args.genome)
     t0 = time.perf_counter()
     rs = care_amd.ReadSet([reads], paired_end=bool(w["options"]["paired_end"]), qual_bits=w["qual_bits"])
     n = rs.n
@@ -357,7 +365,12 @@ def main():
             for name, extra in (("reduced_classic", gate_mod.REDUCED_ARGS),
                                 ("reduced_forest", gate_mod._forest_args(gate_mod.DEFAULT_FOREST,
                                                                          gate_mod.chosen_threshold(gate_mod.DEFAULT_FOREST))),
-                                ("reduced_accept_all", gate_mod._gated_args(gate_path, 1e-6))):
+                                ("reduced_accept_all", gate_mod._gated_args(gate_path, 1e-6)),
+                                ("reduced_mlp_plain_columns",
+                                 gate_mod._gated_args(gate_mod.PLAIN_COLUMNS_GATE,
+                                                      gate_mod.chosen_threshold(gate_mod.PLAIN_COLUMNS_GATE)))):
+                if name == "reduced_mlp_plain_columns" and not os.path.exists(gate_mod.PLAIN_COLUMNS_GATE):
+                    continue
                 if name == "reduced_forest" and not os.path.exists(gate_mod.DEFAULT_FOREST):
                     continue
                 qual[name] = quality(care_amd, reads, truth, w, extra, os.path.join(work, name))

@@ -28,6 +28,9 @@ import care_amd
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_GATE = os.path.join(HERE, "gate_repeat_pe.bin")
+# the same network trained on the plain method's decision columns (every
+# candidate, refinement) instead of the reduced pipeline's
+PLAIN_COLUMNS_GATE = os.path.join(HERE, "gate_plain_columns.bin")
 
 # the paired-end repeat workload's generator and options
 PROFILE = "repeat-pe"
@@ -390,6 +393,9 @@ def ablation(reads, truth, work, gate_path=DEFAULT_GATE, forest_path=DEFAULT_FOR
         rows["reduced_forest"] = _forest_args(forest_path, chosen_threshold(forest_path))
     if os.path.exists(gate_path):
         rows["reduced_mlp_gate"] = _gated_args(gate_path, chosen_threshold(gate_path))
+    if os.path.exists(PLAIN_COLUMNS_GATE):
+        rows["reduced_mlp_gate_plain_columns"] = _gated_args(PLAIN_COLUMNS_GATE, chosen_threshold(PLAIN_COLUMNS_GATE))
+    if os.path.exists(gate_path):
         # a gate that accepts every column: logit threshold far below any output
         rows["reduced_accept_all"] = _gated_args(gate_path, 1e-6)
     res = {}
