@@ -154,7 +154,7 @@ def cpu_baseline(reads, w, sample, threads):
                       "query+correction %.2f s on %d threads, %s" % (sample, float(m.group(1)), threads, model)}
 
 
-def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=True):
+def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=True, graph=True):
     """Average wall time of one correction step over all reads (reads and
     tables resident in HBM), the correction kernel's HIP-event time and
     launch count, and the candidates of the last step."""
@@ -220,7 +220,7 @@ def time_path(care_amd, rs, opt, steps, warmup, gate=None, batch=0, autotune=Tru
     # first call, replayed by the timed ones; results checked against the
     # last eager step
     graph_ms = None
-    if chunk == n:
+    if graph and chunk == n:
         ref_status = out[0][:n].copy()
         eng.correct(0, n, candidate_capacity=cap, out=out, graph=True)
         gms = []
@@ -305,6 +305,7 @@ def main():
     ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replays of the step")
     ap.add_argument("--genome", type=int, default=0,
                     help="genome size of the workload's generator (default: the workload's own)")
     args = ap.parse_args()
@@ -329,7 +330,7 @@ def main():
     log("bench: %d reads packed in %.2f s" % (n, time.perf_counter() - t0))
 
     plain = time_path(care_amd, rs, care_amd.default_options(**w["options"]), args.steps, args.warmup,
-                      batch=args.batch, autotune=not args.no_autotune)
+                      batch=args.batch, autotune=not args.no_autotune, graph=not args.no_graph)
     gated = None
     gate_path = gate_mod.DEFAULT_GATE
     gated_ok = args.workload == "repeat-pe" and os.path.exists(gate_path)
@@ -339,7 +340,7 @@ def main():
                                                 candidate_limit=gate_mod.CANDIDATE_LIMIT,
                                                 threshold_anchor=threshold))
         gated = time_path(care_amd, rs, gopt, args.steps, args.warmup, gate=gate_mod.load(gate_path),
-                          batch=args.batch, autotune=not args.no_autotune)
+                          batch=args.batch, autotune=not args.no_autotune, graph=not args.no_graph)
     gated_large = {}
     if args.workload == "large-pe" and os.path.exists(gate_path):
         # the gate on anchors, and on anchors and candidate corrections
