@@ -1044,6 +1044,10 @@ namespace {
 // context's options (device buffers are filled in by the caller).
 int prepare_params(care_context* ctx, CorrectParams& P) {
     const auto& o = ctx->opt;
+    // MSA base counts are 16-bit: an anchor has at most maxCand candidates
+    if (ctx->maxCand > 65534)
+        return fail(CARE_ERR_ARGUMENT, "%d candidates per anchor (maps x results per map) exceed 65534",
+                    ctx->maxCand);
     P.rs = ctx->rs;
     P.maxCand = ctx->maxCand;
     P.qweights = ctx->qweights.p;

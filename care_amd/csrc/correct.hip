@@ -281,66 +281,93 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
     return msa;
 }
 
+// masks of the two words j, j + 1 of a lane with nw words whose last word
+// keeps the bits of lastMask
+__device__ __forceinline__ void word_masks(int j, int nw, u32 lastMask, u32* m1, u32* m2) {
+    const u32 a = j + 1 < nw ? ~0u : lastMask;
+    const u32 b = j + 2 < nw ? ~0u : lastMask;
+    *m1 = j < nw ? a : 0u;
+    *m2 = j + 1 < nw ? b : 0u;
+}
+
 // Alignment of the anchor against the candidate planes of one wave, all
 // shifts in parallel, both orientations of a shift in the same lane; result
-// key = score << 11 | order index, or ~0u.  For shift s >= 0 the anchor is
-// read from base s against the candidate from base 0, for s < 0 the
-// candidate from base -s against the anchor from base 0: one side is always
-// word aligned, and the anchor side is shared by the two orientations.  The
-// word loop is uniform over the wave and ends when no lane is still below
-// its mismatch limit in either orientation (mismatch counts only grow, so
-// counting past the limit changes no result); reads past a lane's last word
-// stay inside the workgroup's LDS and are masked.
+// key = score << 11 | order index u (u = s for the shifts s >= 0, then
+// u = nPos - 1 - s for s < 0), or ~0u.  The shifts s >= 0 and s < 0 run in
+// separate wave-uniform loops.  For s >= 0 the anchor is read from base s
+// against the candidate from base 0, so the shifted anchor words serve both
+// orientations; for s < 0 the candidate is read from base -s against the
+// anchor from base 0, whose words are the same in every lane.  The word
+// loop ends when no lane is still below its mismatch limit in either
+// orientation (mismatch counts only grow, so counting past the limit
+// changes no result); reads past a lane's last word stay inside the
+// workgroup's LDS and are masked.
 __device__ __forceinline__ void align_wave(const uint2* ap, int la, const uint2* fp, const uint2* rp, int lc,
                                            int minov, float maxErrorRate, u32* keyF, u32* keyR) {
     const int lane = lane_id();
     const int nPos = max(0, la - minov + 1);
     const int nNeg = max(0, lc - minov);
-    const int tot = nPos + nNeg;
     const int total = la + lc;
     u32 kf = ~0u, kr = ~0u;
-    for (int u0 = 0; u0 < tot; u0 += kWave) {
-        const int u = u0 + lane;
-        const bool inRange = u < tot;
-        const int s = u < nPos ? u : -(u - nPos + 1);
-        const int ov = s >= 0 ? min(la - s, lc) : min(la, lc + s);
+    for (int u0 = 0; u0 < nPos; u0 += kWave) {
+        const int s = u0 + lane;  // order index u = s
+        const int ov = min(la - s, lc);
         const int lim = int(float(ov) * maxErrorRate);
-        const int nw = (inRange && lim > 0) ? (ov + 31) >> 5 : 0;
-        const int d = s >= 0 ? s : -s;
-        const u32 sh = u32(32 - (d & 31)) & 31u;
-        const int off = (d >> 5) + ((d & 31) != 0);
-        const uint2* p0 = (s >= 0 ? ap : fp) + off;  // shifted side, forward candidate
-        const uint2* p1 = (s >= 0 ? ap : rp) + off;  // shifted side, reverse-complement candidate
-        const uint2* q0 = (s >= 0 ? fp : ap) + 1;    // word-aligned side
-        const uint2* q1 = (s >= 0 ? rp : ap) + 1;
+        const int nw = (s < nPos && lim > 0) ? (ov + 31) >> 5 : 0;
+        const u32 sh = u32(32 - (s & 31)) & 31u;
+        const uint2* p = ap + (s >> 5) + ((s & 31) != 0);
         const u32 lastMask = (ov & 31) ? ~(~0u >> (ov & 31)) : ~0u;
-        uint2 a0 = p0[0], b0 = p1[0];
         int mf = 0, mr = 0;
-        // two words per iteration: one LDS round trip covers 64 bases
+        // two words per iteration: one LDS round trip covers 64 bases (the
+        // first shifted word is read again rather than carried in registers)
         for (int j = 0; __any(j < nw && (mf < lim || mr < lim)); j += 2) {
-            const uint2 a1 = p0[j + 1], b1 = p1[j + 1], a2 = p0[j + 2], b2 = p1[j + 2];
-            const uint2 va = q0[j], vb = q1[j], va2 = q0[j + 1], vb2 = q1[j + 1];
-            u32 msk = j + 1 < nw ? ~0u : lastMask;
-            msk = j < nw ? msk : 0u;
-            u32 msk2 = j + 2 < nw ? ~0u : lastMask;
-            msk2 = j + 1 < nw ? msk2 : 0u;
-            const u32 xf = (__builtin_amdgcn_alignbit(a0.x, a1.x, sh) ^ va.x) |
-                           (__builtin_amdgcn_alignbit(a0.y, a1.y, sh) ^ va.y);
-            const u32 xr = (__builtin_amdgcn_alignbit(b0.x, b1.x, sh) ^ vb.x) |
-                           (__builtin_amdgcn_alignbit(b0.y, b1.y, sh) ^ vb.y);
-            const u32 xf2 = (__builtin_amdgcn_alignbit(a1.x, a2.x, sh) ^ va2.x) |
-                            (__builtin_amdgcn_alignbit(a1.y, a2.y, sh) ^ va2.y);
-            const u32 xr2 = (__builtin_amdgcn_alignbit(b1.x, b2.x, sh) ^ vb2.x) |
-                            (__builtin_amdgcn_alignbit(b1.y, b2.y, sh) ^ vb2.y);
-            mf += __popc(xf & msk) + __popc(xf2 & msk2);
-            mr += __popc(xr & msk) + __popc(xr2 & msk2);
-            a0 = a2;
-            b0 = b2;
+            const uint2 a0 = p[j], a1 = p[j + 1], a2 = p[j + 2];
+            const uint2 f1 = fp[j + 1], f2 = fp[j + 2], r1 = rp[j + 1], r2 = rp[j + 2];
+            u32 msk, msk2;
+            word_masks(j, nw, lastMask, &msk, &msk2);
+            const u32 h1 = __builtin_amdgcn_alignbit(a0.x, a1.x, sh), l1 = __builtin_amdgcn_alignbit(a0.y, a1.y, sh);
+            const u32 h2 = __builtin_amdgcn_alignbit(a1.x, a2.x, sh), l2 = __builtin_amdgcn_alignbit(a1.y, a2.y, sh);
+            mf += __popc(((h1 ^ f1.x) | (l1 ^ f1.y)) & msk) + __popc(((h2 ^ f2.x) | (l2 ^ f2.y)) & msk2);
+            mr += __popc(((h1 ^ r1.x) | (l1 ^ r1.y)) & msk) + __popc(((h2 ^ r2.x) | (l2 ^ r2.y)) & msk2);
         }
         if (nw > 0) {
             const u32 base = u32(total - 2 * ov);
-            if (mf < lim) kf = min(kf, ((u32(mf) + base) << 11) | u32(u));
-            if (mr < lim) kr = min(kr, ((u32(mr) + base) << 11) | u32(u));
+            if (mf < lim) kf = min(kf, ((u32(mf) + base) << 11) | u32(s));
+            if (mr < lim) kr = min(kr, ((u32(mr) + base) << 11) | u32(s));
+        }
+    }
+    for (int v0 = 0; v0 < nNeg; v0 += kWave) {
+        const int d = v0 + lane + 1;  // shift s = -d, order index u = nPos - 1 + d
+        const int ov = min(la, lc - d);
+        const int lim = int(float(ov) * maxErrorRate);
+        const int nw = (d <= nNeg && lim > 0) ? (ov + 31) >> 5 : 0;
+        const u32 sh = u32(32 - (d & 31)) & 31u;
+        const int off = (d >> 5) + ((d & 31) != 0);
+        const uint2* pf = fp + off;
+        const uint2* pr = rp + off;
+        const u32 lastMask = (ov & 31) ? ~(~0u >> (ov & 31)) : ~0u;
+        int mf = 0, mr = 0;
+        for (int j = 0; __any(j < nw && (mf < lim || mr < lim)); j += 2) {
+            const uint2 f0 = pf[j], f1 = pf[j + 1], f2 = pf[j + 2], r0 = pr[j], r1 = pr[j + 1], r2 = pr[j + 2];
+            const uint2 a1 = ap[j + 1], a2 = ap[j + 2];
+            u32 msk, msk2;
+            word_masks(j, nw, lastMask, &msk, &msk2);
+            const u32 xf = (__builtin_amdgcn_alignbit(f0.x, f1.x, sh) ^ a1.x) |
+                           (__builtin_amdgcn_alignbit(f0.y, f1.y, sh) ^ a1.y);
+            const u32 xr = (__builtin_amdgcn_alignbit(r0.x, r1.x, sh) ^ a1.x) |
+                           (__builtin_amdgcn_alignbit(r0.y, r1.y, sh) ^ a1.y);
+            const u32 xf2 = (__builtin_amdgcn_alignbit(f1.x, f2.x, sh) ^ a2.x) |
+                            (__builtin_amdgcn_alignbit(f1.y, f2.y, sh) ^ a2.y);
+            const u32 xr2 = (__builtin_amdgcn_alignbit(r1.x, r2.x, sh) ^ a2.x) |
+                            (__builtin_amdgcn_alignbit(r1.y, r2.y, sh) ^ a2.y);
+            mf += __popc(xf & msk) + __popc(xf2 & msk2);
+            mr += __popc(xr & msk) + __popc(xr2 & msk2);
+        }
+        if (nw > 0) {
+            const u32 base = u32(total - 2 * ov);
+            const u32 u = u32(nPos - 1 + d);
+            if (mf < lim) kf = min(kf, ((u32(mf) + base) << 11) | u);
+            if (mr < lim) kr = min(kr, ((u32(mr) + base) << 11) | u);
         }
     }
     *keyF = wave_min_u32(kf);
@@ -821,11 +848,11 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 L.cpl[size_t(2 * c + 1) * pw + p] = make_uint2(hi, lo);
             }
             __syncthreads();
-            // the plain kernel decodes the keys and chooses orientations for
-            // the whole chunk at once, one thread per candidate (its chunks
-            // hold about 50 candidates); the gated one (at most candidate_limit
-            // candidates, a few per wave) decodes in each wave's lane 0 and
-            // saves the extra barrier
+            // each wave aligns a candidate at a time and leaves its two keys
+            // in LDS; the keys of the whole chunk are then decoded and the
+            // orientations chosen one thread per candidate (decoding in the
+            // aligning wave's lane 0 would issue every instruction for the
+            // whole wave)
             for (int c = wave; c < nc; c += nwaves) {
                 const int lc = L.mst[c].len;
                 const uint2* fp = L.cpl + size_t(2 * c) * pw;
@@ -834,20 +861,14 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                 align_wave(L.ap, la, fp, rp, lc, minovA, P.maxErrorRate, &kf, &kr);
                 if (lane == 0) {
                     CARE_CHECK(c0 + c < n);
-                    if (kGated) {
-                        metaA[c0 + c] = aligned_meta(L.mst[c], kf, kr, la, minovA, P);
-                    } else {
-                        L.keys[2 * c] = kf;
-                        L.keys[2 * c + 1] = kr;
-                    }
+                    L.keys[2 * c] = kf;
+                    L.keys[2 * c + 1] = kr;
                 }
                 wave_sync();
             }
-            if (!kGated) {
-                __syncthreads();
-                for (int c = tid; c < nc; c += blockDim.x)
-                    metaA[c0 + c] = aligned_meta(L.mst[c], L.keys[2 * c], L.keys[2 * c + 1], la, minovA, P);
-            }
+            __syncthreads();
+            for (int c = tid; c < nc; c += blockDim.x)
+                metaA[c0 + c] = aligned_meta(L.mst[c], L.keys[2 * c], L.keys[2 * c + 1], la, minovA, P);
         }
         __syncthreads();
 
