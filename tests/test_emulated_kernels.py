@@ -36,6 +36,9 @@ SMALL = {
     "pe_reduced_small": {"profile": "repeat-pe", "seed": 205, "genome": 2000,
                          "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "2", "-k", "20", "-h",
                                   "32", "--candidateLimit", "20", "--refinements", "0"]},
+    # 400 bp reads: alignment windows longer than the unrolled word steps
+    "se_long_small": {"profile": "uniform-se", "seed": 207, "genome": 3000, "sim_args": ["--length", "400"],
+                      "args": ["-c", "30", "--pairmode", "SE", "-k", "20", "-h", "16", "--candidateCorrection"]},
     "pe_forest_small": {"profile": "repeat-pe", "seed": 204, "genome": 1500,
                         "args": ["-c", "30", "--pairmode", "PE", "-q", "-k", "20", "-h", "32",
                                  "--candidateCorrection", "--correctionType", "1", "--correctionTypeCands", "1"],
