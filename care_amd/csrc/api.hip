@@ -1131,6 +1131,15 @@ void key_append(std::vector<unsigned char>& key, const T&... vals) {
     (one(vals), ...);
 }
 
+// Size of the next device batch of a call over n anchors with rem still to
+// go: batches of B, except that a call of more than B anchors ends in a
+// batch of B / 8, so the result copy that no later batch overlaps is short.
+static int next_batch(int B, int n, int rem) {
+    const int tail = (B / 8) & ~1;
+    if (n > B && tail >= 2 && rem > tail && rem <= B + tail) return rem - tail;
+    return std::min(B, rem);
+}
+
 int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_result* r, bool graph) {
     if (!ctx || !r || !r->anchor_status || !r->anchor_sequences) return fail(CARE_ERR_ARGUMENT, "null argument");
     if (!ctx->hasTables) return fail(CARE_ERR_STATE, "tables not built");
@@ -1176,7 +1185,7 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
         // no host synchronisation and the results come back in a few copies
         // at the end.  Candidate results come in no particular order: the
         // per-read combination of results does not depend on it.
-        const int nBatches = (n + B - 1) / B;
+        const int nBatches = (n + B - 1) / B + 1;  // at most, with the short final batch
         if (n > ctx->outCap) {
             ctx->outStatus.alloc(size_t(n));
             ctx->outSeq.alloc(size_t(n) * ctx->outPitch);
@@ -1230,8 +1239,8 @@ int correct_batch_impl(care_context* ctx, uint64_t first, int n, care_batch_resu
             try {
                 launch_fill_u64(reinterpret_cast<u64*>(ctx->rawHits.p), 2, 0, s);  // totals over the call
                 launch_reset_counters(ctx->counters.p, ctx->counters.p + 1, s);  // candidate results over the call
-                for (int done = 0; done < n; done += B, bi++) {
-                    const int nb = std::min(B, n - done);
+                for (int done = 0, nb = 0; done < n; done += nb, bi++) {
+                    nb = next_batch(B, n, n - done);
                     hipEvent_t* e = &ctx->batchEv[size_t(3 * bi)];
                     if (!graph) HIP_TRY(hipEventRecord(e[0], s));
                     launch_query(ctx->rs, ctx->tables, first + done, nb, ctx->maxCand, ctx->queryCap,
