@@ -497,19 +497,34 @@ __device__ __forceinline__ void sort_in_registers(u32 (&v)[K], int lane) {
 template <int K, class Buf>
 __device__ __forceinline__ void gather_sort(Buf out, const int* counts, const u32* const* lists, int h, int total,
                                             int lane) {
+    // a lane's K entries are consecutive: one binary search finds the list
+    // of its first entry, the others step through the following lists
+    // (counts[h] = total ends the walk)
+    const int e0 = lane * K;
+    int j = 0;
+    if (e0 < total) {
+        int hi = h - 1;  // last list j with counts[j] <= e0
+        while (j < hi) {
+            const int mid = (j + hi + 1) >> 1;
+            if (counts[mid] <= e0) j = mid;
+            else hi = mid - 1;
+        }
+    }
+    int start = counts[j], next = counts[j + 1];
+    const u32* list = lists[j];
     u32 v[K];
 #pragma unroll
     for (int r = 0; r < K; r++) {
-        const int e = lane * K + r;
+        const int e = e0 + r;
         u32 x = 0xFFFFFFFFu;
         if (e < total) {
-            int lo = 0, hi = h - 1;  // last list j with counts[j] <= e
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (counts[mid] <= e) lo = mid;
-                else hi = mid - 1;
+            while (e >= next) {
+                j++;
+                start = next;
+                next = counts[j + 1];
+                list = lists[j];
             }
-            x = lists[lo][e - counts[lo]];
+            x = list[e - start];
         }
         v[r] = x;
     }
@@ -544,13 +559,19 @@ __device__ __forceinline__ int compact_unique(Buf sorted, int total, u32 self, c
                                               int excludeAmbiguous, int candLimit, u32* out, HitBuf hits, int lane) {
     int written = 0;
     const u64 below = lanes_below(lane);
+    // hit counts (candLimit > 0): the maps that hit v are the length of its
+    // run of equal entries (at most h <= 64), which ends where the next run
+    // starts; the last kept run of a chunk waits in (pendPos, pendStart)
+    // until a later chunk's first run start, or the end of the list
+    int pendPos = -1, pendStart = 0;
     for (int base = 0; base < total; base += kWave) {
         const int i = base + lane;
-        bool keep = false;
+        bool start = false, keep = false;
         u32 v = 0;
         if (i < total) {
             v = sorted[i];
-            keep = (i == 0 || sorted[i - 1] != v) && v != self;
+            start = i == 0 || sorted[i - 1] != v;
+            keep = start && v != self;
             if (keep && excludeAmbiguous) keep = rs.ambig[v] == 0;
         }
         const u64 bal = __ballot(keep);
@@ -558,19 +579,26 @@ __device__ __forceinline__ int compact_unique(Buf sorted, int total, u32 self, c
         if (keep) {
             CARE_CHECK(v < rs.n && pos < total);
             out[pos] = v;
-            if (candLimit > 0) {
-                // maps that hit v: length of its run of equal entries (at most
-                // h <= 64), found by binary lifting
-                int run = 1;
-                for (int step = 32; step > 0; step >>= 1) {
-                    const int t = run + step;
-                    if (t <= 64 && i + t - 1 < total && sorted[i + t - 1] == v) run = t;
+        }
+        if (candLimit > 0) {
+            const u64 starts = __ballot(start);  // the same in every lane
+            if (starts) {
+                if (pendPos >= 0) {
+                    if (lane == 0) hits[pendPos] = u8(base + __ffsll((long long)starts) - 1 - pendStart);
+                    pendPos = -1;
                 }
-                hits[pos] = u8(run);
+                const u64 after = starts & ~(below | (1ull << lane));  // run starts after this lane's
+                if (keep && after) hits[pos] = u8(base + __ffsll((long long)after) - 1 - i);
+                const int last = 63 - __clzll((long long)starts);
+                if ((bal >> last) & 1ull) {
+                    pendPos = written + __popcll(bal & ((1ull << last) - 1ull));
+                    pendStart = base + last;
+                }
             }
         }
         written += __popcll(bal);
     }
+    if (pendPos >= 0 && lane == 0) hits[pendPos] = u8(total - pendStart);
     return written;
 }
 

@@ -499,6 +499,7 @@ inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz(unsigned(x)); }
+inline int __clzll(long long x) { return x == 0 ? 64 : __builtin_clzll((unsigned long long)x); }
 inline unsigned __brev(unsigned x) {
     unsigned r = 0;
     for (int i = 0; i < 32; i++) r |= ((x >> i) & 1u) << (31 - i);
