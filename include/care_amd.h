@@ -272,7 +272,8 @@ int care_set_launch_config(care_context* ctx, int batch_anchors, int block_threa
  * counterpart in the reference, which fixes its launch shapes at compile
  * time (src/gpu/alignmentkernels.cu:652, src/gpu/msakernels.cu:1186). */
 typedef struct care_tuning {
-    int batch_anchors;          /* anchors per internal launch, even (0 = automatic) */
+    int batch_anchors;          /* anchors per internal launch, even (0 = automatic); a call of
+                                   more than one launch ends in one of batch_anchors / 8 */
     int correct_threads;        /* threads per correction workgroup, i.e. 2, 3 or 4
                                    wavefronts per anchor: 128, 192 or 256 (0 = 256) */
     int correct_blocks_per_cu;  /* persistent correction workgroups per compute unit
