@@ -133,6 +133,26 @@ def test_reduced_pipeline_matches_oracle(tmp_path):
     assert helpers.md5(str(tmp_path / "g" / "g.fastq")) == helpers.md5(str(tmp_path / "o" / "o.fastq"))
 
 
+def test_long_reads_match_oracle(tmp_path):
+    """300-base paired-end reads (alignment windows of 10 words, MSA windows
+    of 900 columns) with 2-bit qualities and candidate correction: the
+    device output equals the oracle's."""
+    case = {"profile": "repeat-pe", "seed": 232, "genome": 30000, "sim_args": ["--length", "300"],
+            "args": ["-c", "30", "--pairmode", "PE", "-q", "--qualityScoreBits", "2", "-k", "20", "-h", "32",
+                     "--candidateCorrection"]}
+    _say("300-base reads vs oracle")
+    reads, _ = helpers.simulate(case, str(tmp_path))
+    import subprocess
+    gpu = care_amd.run_care_gpu(["-i", reads, "-o", "g.fastq", "-d", str(tmp_path / "g"), "-g", "0", "-t", "8"]
+                                + case["args"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                                timeout=300)
+    assert gpu.returncode == 0, gpu.stdout[-2000:]
+    orc = subprocess.run([helpers.ORACLE_EXE, "-i", reads, "-o", "o.fastq", "-d", str(tmp_path / "o"), "-t", "8"]
+                         + case["args"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert orc.returncode == 0, orc.stdout[-2000:]
+    assert helpers.md5(str(tmp_path / "g" / "g.fastq")) == helpers.md5(str(tmp_path / "o" / "o.fastq"))
+
+
 def _split_gz_mates(reads, d):
     """The interleaved mates of `reads` as two gzip'ed files."""
     import gzip
