@@ -53,6 +53,11 @@ __device__ __forceinline__ u32 even_bits(u32 x) {
     return x;
 }
 
+// Shifts still under their mismatch limit after the first 64 bases, queued
+// per wavefront: (candidate << 11 | order index, mismatches forward | reverse
+// << 16).  A slice holds one wavefront's worth.
+constexpr int kSurvivors = 64;
+
 // LDS of one workgroup.  A sequence is held as interleaved bit planes: word
 // k is (hi bits, lo bits) of bases 32k..32k+31, MSB first.  Planes carry one
 // leading and two trailing zero words: plane word i lives at index i + 1, so
@@ -66,7 +71,8 @@ struct Lds {
     u32* aw;        // anchor 2-bit words [pitch]
     uint2* ap;      // anchor planes [hw + 3]
     char* anchorQ;  // [maxLen]
-    u32* keys;      // [chunk][2] alignment keys (forward, reverse complement) of the staged candidates
+    u32* keys;      // [chunk][2] alignment keys (forward, reverse complement) of the staged candidates,
+                    // followed by the survivor slices of the waves [waves][2][kSurvivors]
     uint2* cpl;     // [chunk] candidate planes: forward, then reverse complement, 2 x [hw + 3]
     u8* cons;       // [cols]
     float* support;
@@ -96,6 +102,7 @@ struct LdsDims {
         b += 4 * sizeof(u16) * cols + 4 * sizeof(float) * cols;
         b += 32 * sizeof(int);
         b += 2 * sizeof(u32) * size_t(chunk);
+        b += 2 * sizeof(u32) * size_t(kSurvivors) * size_t(waves);  // survivor slices after the keys
         if (gated & 1) b += (sizeof(u16) + 1) * ((size_t(maxLen) + 15) / 16 * 16);
         return b;
     }
@@ -121,6 +128,7 @@ __device__ Lds carve(char* base, const LdsDims& d) {
     l.cons = reinterpret_cast<u8*>(p); p += (size_t(d.cols) + 3) / 4 * 4;
     l.anchorQ = reinterpret_cast<char*>(p); p += (size_t(d.maxLen) + 3) / 4 * 4;
     l.keys = reinterpret_cast<u32*>(p); p += 2 * sizeof(u32) * size_t(d.chunk);
+    p += 2 * sizeof(u32) * size_t(kSurvivors) * size_t(d.waves);  // survivor slices
     l.gscr = reinterpret_cast<char*>(l.mst);
     if (d.gated & 1) {
         const size_t ml = (size_t(d.maxLen) + 15) / 16 * 16;
@@ -290,56 +298,147 @@ __device__ __forceinline__ void word_masks(int j, int nw, u32 lastMask, u32* m1,
     *m2 = j + 1 < nw ? b : 0u;
 }
 
-// Alignment of the anchor against the candidate planes of one wave, all
-// shifts in parallel, both orientations of a shift in the same lane; result
-// key = score << 11 | order index u (u = s for the shifts s >= 0, then
-// u = nPos - 1 - s for s < 0), or ~0u.  The shifts s >= 0 and s < 0 run in
-// separate wave-uniform loops.  For s >= 0 the anchor is read from base s
-// against the candidate from base 0, so the shifted anchor words serve both
-// orientations; for s < 0 the candidate is read from base -s against the
-// anchor from base 0, whose words are the same in every lane.  The word
-// loop ends when no lane is still below its mismatch limit in either
-// orientation (mismatch counts only grow, so counting past the limit
-// changes no result); reads past a lane's last word stay inside the
-// workgroup's LDS and are masked.
-__device__ __forceinline__ void align_wave(const uint2* ap, int la, const uint2* fp, const uint2* rp, int lc,
-                                           int minov, float maxErrorRate, u32* keyF, u32* keyR) {
+struct AlignCtx {
+    const uint2* ap;   // anchor planes
+    const uint2* cpl;  // candidate planes of the chunk, 2 x pw per candidate
+    const CandMeta* mst;
+    u32* keys;         // [chunk][2] keys, combined by atomic minimum
+    u32* survA;        // this wave's survivor slice
+    u32* survB;
+    int la, pw, minov;
+    float maxErrorRate;
+};
+
+// Finishes the queued survivors of a wavefront (one per lane): the words
+// from 64 on, then the keys into the candidates' minima.  All lanes call.
+__device__ __forceinline__ void finish_survivors(const AlignCtx& A, int count) {
     const int lane = lane_id();
-    const int nPos = max(0, la - minov + 1);
-    const int nNeg = max(0, lc - minov);
+    const bool valid = lane < count;
+    u32 ea = 0, eb = 0;
+    if (valid) {
+        ea = A.survA[lane];
+        eb = A.survB[lane];
+    }
+    const int c = int(ea >> 11), u = int(ea & 2047u);
+    int mf = int(eb & 0xFFFFu), mr = int(eb >> 16);
+    const int lc = valid ? int(A.mst[c].len) : 0;
+    const int la = A.la;
+    const int nPos = max(0, la - A.minov + 1);
+    const bool pos = u < nPos;
+    const int s = pos ? u : -(u - nPos + 1);
+    const int ov = valid ? (pos ? min(la - s, lc) : min(la, lc + s)) : 0;
+    const int lim = int(float(ov) * A.maxErrorRate);
+    const int nw = valid ? (ov + 31) >> 5 : 0;
+    const int d = pos ? s : -s;
+    const u32 sh = u32(32 - (d & 31)) & 31u;
+    const int off = (d >> 5) + ((d & 31) != 0);
+    const uint2* fp = A.cpl + size_t(2 * c) * A.pw;
+    const uint2* rp = fp + A.pw;
+    const uint2* p0 = (pos ? A.ap : fp) + off;  // shifted side, forward candidate
+    const uint2* p1 = (pos ? A.ap : rp) + off;  // shifted side, reverse-complement candidate
+    const uint2* q0 = (pos ? fp : A.ap) + 1;    // word-aligned side
+    const uint2* q1 = (pos ? rp : A.ap) + 1;
+    const u32 lastMask = (ov & 31) ? ~(~0u >> (ov & 31)) : ~0u;
+    // one word per iteration: the lanes are mostly full here, and fewer
+    // words in flight keep the registers of the calling kernels
+    for (int j = 2; __any(j < nw && (mf < lim || mr < lim)); j++) {
+        const uint2 a0 = p0[j], a1 = p0[j + 1], b0 = p1[j], b1 = p1[j + 1];
+        const uint2 va = q0[j], vb = q1[j];
+        const u32 msk = j + 1 < nw ? ~0u : (j < nw ? lastMask : 0u);
+        const u32 xf = (__builtin_amdgcn_alignbit(a0.x, a1.x, sh) ^ va.x) |
+                       (__builtin_amdgcn_alignbit(a0.y, a1.y, sh) ^ va.y);
+        const u32 xr = (__builtin_amdgcn_alignbit(b0.x, b1.x, sh) ^ vb.x) |
+                       (__builtin_amdgcn_alignbit(b0.y, b1.y, sh) ^ vb.y);
+        mf += __popc(xf & msk);
+        mr += __popc(xr & msk);
+    }
+    if (valid) {
+        const u32 base = u32(la + lc - 2 * ov);
+        if (mf < lim) atomicMin(&A.keys[2 * c], ((u32(mf) + base) << 11) | u32(u));
+        if (mr < lim) atomicMin(&A.keys[2 * c + 1], ((u32(mr) + base) << 11) | u32(u));
+    }
+    wave_sync();
+}
+
+// appends the lanes in `survive` (their order index u and mismatch counts)
+// to the wave's survivor slice, finishing the queued ones first when the
+// slice would overflow; `count` is wave-uniform
+__device__ __forceinline__ void queue_survivors(const AlignCtx& A, int c, bool survive, int u, int mf, int mr,
+                                                int& count) {
+    const u64 bal = __ballot(survive);
+    if (!bal) return;
+    const int k = __popcll(bal);
+    if (count + k > kSurvivors) {
+        finish_survivors(A, count);
+        count = 0;
+    }
+    const int lane = lane_id();
+    if (survive) {
+        const int slot = count + __popcll(bal & (lane == 0 ? 0ull : (~0ull >> (64 - lane))));
+        A.survA[slot] = (u32(c) << 11) | u32(u);
+        A.survB[slot] = u32(mf) | (u32(mr) << 16);
+    }
+    count += k;
+    wave_sync();
+}
+
+// Alignment of the anchor against the candidate planes of one wave, all
+// shifts in parallel, both orientations of a shift in the same lane; key =
+// score << 11 | order index u (u = s for the shifts s >= 0, then
+// u = nPos - 1 - s for s < 0), combined into keys[2c], keys[2c + 1] by
+// atomic minimum (both set to ~0u by the caller first).  The shifts s >= 0
+// and s < 0 run in separate wave-uniform loops.  For s >= 0 the anchor is
+// read from base s against the candidate from base 0, so the shifted anchor
+// words serve both orientations; for s < 0 the candidate is read from base
+// -s against the anchor from base 0, whose words are the same in every
+// lane.  Every shift compares its first 64 bases here; the few shifts still
+// under their mismatch limit (in either orientation) with more bases to go
+// are queued and finished later a wavefront at a time (finish_survivors),
+// instead of holding a wavefront of mostly finished lanes.  Mismatch counts
+// only grow, so counting past the limit changes no result; reads past a
+// lane's last word stay inside the workgroup's LDS and are masked.
+__device__ __forceinline__ void align_wave(const AlignCtx& A, int c, int lc, int& count) {
+    const int lane = lane_id();
+    const int la = A.la;
+    const int nPos = max(0, la - A.minov + 1);
+    const int nNeg = max(0, lc - A.minov);
     const int total = la + lc;
+    const uint2* ap = A.ap;
+    const uint2* fp = A.cpl + size_t(2 * c) * A.pw;
+    const uint2* rp = fp + A.pw;
     u32 kf = ~0u, kr = ~0u;
     for (int u0 = 0; u0 < nPos; u0 += kWave) {
         const int s = u0 + lane;  // order index u = s
         const int ov = min(la - s, lc);
-        const int lim = int(float(ov) * maxErrorRate);
+        const int lim = int(float(ov) * A.maxErrorRate);
         const int nw = (s < nPos && lim > 0) ? (ov + 31) >> 5 : 0;
         const u32 sh = u32(32 - (s & 31)) & 31u;
         const uint2* p = ap + (s >> 5) + ((s & 31) != 0);
         const u32 lastMask = (ov & 31) ? ~(~0u >> (ov & 31)) : ~0u;
         int mf = 0, mr = 0;
-        // two words per iteration: one LDS round trip covers 64 bases (the
-        // first shifted word is read again rather than carried in registers)
-        for (int j = 0; __any(j < nw && (mf < lim || mr < lim)); j += 2) {
-            const uint2 a0 = p[j], a1 = p[j + 1], a2 = p[j + 2];
-            const uint2 f1 = fp[j + 1], f2 = fp[j + 2], r1 = rp[j + 1], r2 = rp[j + 2];
+        // the first two words: one LDS round trip covers 64 bases
+        if (__any(nw > 0)) {
+            const uint2 a0 = p[0], a1 = p[1], a2 = p[2];
+            const uint2 f1 = fp[1], f2 = fp[2], r1 = rp[1], r2 = rp[2];
             u32 msk, msk2;
-            word_masks(j, nw, lastMask, &msk, &msk2);
+            word_masks(0, nw, lastMask, &msk, &msk2);
             const u32 h1 = __builtin_amdgcn_alignbit(a0.x, a1.x, sh), l1 = __builtin_amdgcn_alignbit(a0.y, a1.y, sh);
             const u32 h2 = __builtin_amdgcn_alignbit(a1.x, a2.x, sh), l2 = __builtin_amdgcn_alignbit(a1.y, a2.y, sh);
             mf += __popc(((h1 ^ f1.x) | (l1 ^ f1.y)) & msk) + __popc(((h2 ^ f2.x) | (l2 ^ f2.y)) & msk2);
             mr += __popc(((h1 ^ r1.x) | (l1 ^ r1.y)) & msk) + __popc(((h2 ^ r2.x) | (l2 ^ r2.y)) & msk2);
         }
-        if (nw > 0) {
+        const bool survive = nw > 2 && (mf < lim || mr < lim);
+        if (nw > 0 && !survive) {
             const u32 base = u32(total - 2 * ov);
             if (mf < lim) kf = min(kf, ((u32(mf) + base) << 11) | u32(s));
             if (mr < lim) kr = min(kr, ((u32(mr) + base) << 11) | u32(s));
         }
+        queue_survivors(A, c, survive, s, mf, mr, count);
     }
     for (int v0 = 0; v0 < nNeg; v0 += kWave) {
         const int d = v0 + lane + 1;  // shift s = -d, order index u = nPos - 1 + d
         const int ov = min(la, lc - d);
-        const int lim = int(float(ov) * maxErrorRate);
+        const int lim = int(float(ov) * A.maxErrorRate);
         const int nw = (d <= nNeg && lim > 0) ? (ov + 31) >> 5 : 0;
         const u32 sh = u32(32 - (d & 31)) & 31u;
         const int off = (d >> 5) + ((d & 31) != 0);
@@ -347,11 +446,11 @@ __device__ __forceinline__ void align_wave(const uint2* ap, int la, const uint2*
         const uint2* pr = rp + off;
         const u32 lastMask = (ov & 31) ? ~(~0u >> (ov & 31)) : ~0u;
         int mf = 0, mr = 0;
-        for (int j = 0; __any(j < nw && (mf < lim || mr < lim)); j += 2) {
-            const uint2 f0 = pf[j], f1 = pf[j + 1], f2 = pf[j + 2], r0 = pr[j], r1 = pr[j + 1], r2 = pr[j + 2];
-            const uint2 a1 = ap[j + 1], a2 = ap[j + 2];
+        if (__any(nw > 0)) {
+            const uint2 f0 = pf[0], f1 = pf[1], f2 = pf[2], r0 = pr[0], r1 = pr[1], r2 = pr[2];
+            const uint2 a1 = ap[1], a2 = ap[2];
             u32 msk, msk2;
-            word_masks(j, nw, lastMask, &msk, &msk2);
+            word_masks(0, nw, lastMask, &msk, &msk2);
             const u32 xf = (__builtin_amdgcn_alignbit(f0.x, f1.x, sh) ^ a1.x) |
                            (__builtin_amdgcn_alignbit(f0.y, f1.y, sh) ^ a1.y);
             const u32 xr = (__builtin_amdgcn_alignbit(r0.x, r1.x, sh) ^ a1.x) |
@@ -363,15 +462,22 @@ __device__ __forceinline__ void align_wave(const uint2* ap, int la, const uint2*
             mf += __popc(xf & msk) + __popc(xf2 & msk2);
             mr += __popc(xr & msk) + __popc(xr2 & msk2);
         }
-        if (nw > 0) {
+        const int u = nPos - 1 + d;
+        const bool survive = nw > 2 && (mf < lim || mr < lim);
+        if (nw > 0 && !survive) {
             const u32 base = u32(total - 2 * ov);
-            const u32 u = u32(nPos - 1 + d);
-            if (mf < lim) kf = min(kf, ((u32(mf) + base) << 11) | u);
-            if (mr < lim) kr = min(kr, ((u32(mr) + base) << 11) | u);
+            if (mf < lim) kf = min(kf, ((u32(mf) + base) << 11) | u32(u));
+            if (mr < lim) kr = min(kr, ((u32(mr) + base) << 11) | u32(u));
         }
+        queue_survivors(A, c, survive, u, mf, mr, count);
     }
-    *keyF = wave_min_u32(kf);
-    *keyR = wave_min_u32(kr);
+    kf = wave_min_u32(kf);
+    kr = wave_min_u32(kr);
+    if (lane == 0) {
+        atomicMin(&A.keys[2 * c], kf);
+        atomicMin(&A.keys[2 * c + 1], kr);
+    }
+    wave_sync();
 }
 
 struct Aln {
@@ -853,18 +959,29 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             // orientations chosen one thread per candidate (decoding in the
             // aligning wave's lane 0 would issue every instruction for the
             // whole wave)
-            for (int c = wave; c < nc; c += nwaves) {
-                const int lc = L.mst[c].len;
-                const uint2* fp = L.cpl + size_t(2 * c) * pw;
-                const uint2* rp = fp + pw;
-                u32 kf, kr;
-                align_wave(L.ap, la, fp, rp, lc, minovA, P.maxErrorRate, &kf, &kr);
-                if (lane == 0) {
-                    CARE_CHECK(c0 + c < n);
-                    L.keys[2 * c] = kf;
-                    L.keys[2 * c + 1] = kr;
+            {
+                AlignCtx A;
+                A.ap = L.ap;
+                A.cpl = L.cpl;
+                A.mst = L.mst;
+                A.keys = L.keys;
+                A.survA = L.keys + 2 * dims.chunk + 2 * kSurvivors * wave;
+                A.survB = A.survA + kSurvivors;
+                A.la = la;
+                A.pw = pw;
+                A.minov = minovA;
+                A.maxErrorRate = P.maxErrorRate;
+                int count = 0;  // queued survivors of this wave (wave-uniform)
+                for (int c = wave; c < nc; c += nwaves) {
+                    if (lane == 0) {
+                        CARE_CHECK(c0 + c < n);
+                        L.keys[2 * c] = ~0u;
+                        L.keys[2 * c + 1] = ~0u;
+                    }
+                    wave_sync();
+                    align_wave(A, c, L.mst[c].len, count);
                 }
-                wave_sync();
+                if (count > 0) finish_survivors(A, count);
             }
             __syncthreads();
             for (int c = tid; c < nc; c += blockDim.x)

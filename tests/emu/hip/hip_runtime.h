@@ -517,6 +517,11 @@ inline int atomicMin(int* p, int v) {
     while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
     return cur;
 }
+inline unsigned atomicMin(unsigned* p, unsigned v) {
+    unsigned cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
+    return cur;
+}
 inline int atomicMax(int* p, int v) {
     int cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
     while (v > cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
