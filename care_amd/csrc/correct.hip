@@ -428,7 +428,8 @@ __device__ __forceinline__ void align_wave(const AlignCtx& A, int c, int lc, int
             mr += __popc(((h1 ^ r1.x) | (l1 ^ r1.y)) & msk) + __popc(((h2 ^ r2.x) | (l2 ^ r2.y)) & msk2);
         }
         const bool survive = nw > 2 && (mf < lim || mr < lim);
-        if (nw > 0 && !survive) {
+        // finished under the limit (rare: most shifts end over it)
+        if (nw > 0 && !survive && (mf < lim || mr < lim)) {
             const u32 base = u32(total - 2 * ov);
             if (mf < lim) kf = min(kf, ((u32(mf) + base) << 11) | u32(s));
             if (mr < lim) kr = min(kr, ((u32(mr) + base) << 11) | u32(s));
@@ -464,7 +465,7 @@ __device__ __forceinline__ void align_wave(const AlignCtx& A, int c, int lc, int
         }
         const int u = nPos - 1 + d;
         const bool survive = nw > 2 && (mf < lim || mr < lim);
-        if (nw > 0 && !survive) {
+        if (nw > 0 && !survive && (mf < lim || mr < lim)) {
             const u32 base = u32(total - 2 * ov);
             if (mf < lim) kf = min(kf, ((u32(mf) + base) << 11) | u32(u));
             if (mr < lim) kr = min(kr, ((u32(mr) + base) << 11) | u32(u));
