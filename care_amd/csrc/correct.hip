@@ -472,12 +472,10 @@ __device__ __forceinline__ void align_wave(const AlignCtx& A, int c, int lc, int
         }
         queue_survivors(A, c, survive, u, mf, mr, count);
     }
-    kf = wave_min_u32(kf);
-    kr = wave_min_u32(kr);
-    if (lane == 0) {
-        atomicMin(&A.keys[2 * c], kf);
-        atomicMin(&A.keys[2 * c + 1], kr);
-    }
+    // few lanes hold a key here, so each adds its own instead of a
+    // wavefront reduction per candidate
+    if (kf != ~0u) atomicMin(&A.keys[2 * c], kf);
+    if (kr != ~0u) atomicMin(&A.keys[2 * c + 1], kr);
     wave_sync();
 }
 
