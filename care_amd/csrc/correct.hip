@@ -160,6 +160,66 @@ struct Msa {
 // new_columns of the anchor and the gate features two columns around a
 // corrected read) stays inside them, and columns are independent of each
 // other.  All threads must call.
+// per-column sums of the MSA: base counts and weight sums, the weights
+// added in candidate order as the CPU adds them
+struct MsaSums {
+    int cA, cC, cG, cT;
+    float wA, wC, wG, wT;
+};
+
+// conditional adds of +0.0f leave the other sums bit-identical (no sum is
+// ever -0.0f); base 4 adds nothing
+__device__ __forceinline__ void msa_add(MsaSums& s, u32 b, float w) {
+    s.cA += b == 0u;
+    s.cC += b == 1u;
+    s.cG += b == 2u;
+    s.cT += b == 3u;
+    s.wA += b == 0u ? w : 0.0f;
+    s.wC += b == 1u ? w : 0.0f;
+    s.wG += b == 2u ? w : 0.0f;
+    s.wT += b == 3u ? w : 0.0f;
+}
+
+// adds the staged candidates 0..nc-1 to the sums of MSA column colRel
+// (relative to the anchor's first base).  kQ is the quality width (0: no
+// qualities, else 1, 2 or 8 bits per base); q4 holds the weights of the
+// 1- and 2-bit bins.  The body has no branch: the orientation selects
+// indices, not code paths.  Left rolled: unrolled copies hold more loads in
+// flight and raise the registers of the calling kernels.
+template <int kQ>
+__device__ __forceinline__ void msa_add_staged(const Lds& L, int nc, int colRel, int pitch, int qpitch, int maxLen,
+                                               const float (&q4)[4], MsaSums& s) {
+#pragma unroll 1
+    for (int c = 0; c < nc; c++) {
+        const CandMeta cm = L.mst[c];
+        const int len = cm.len;
+        const int p0 = colRel - int(cm.shift);
+        const bool in = u32(p0) < u32(len);
+        const int pos = in ? p0 : 0;
+        const bool rc = cm.flag == 2;
+        // reverse complement: base len-1-pos of the stored read, complemented
+        const int i = rc ? len - 1 - pos : pos;
+        const u32 word = L.rows[c * pitch + (i >> 4)];
+        const u32 b = ((word >> (30 - 2 * (i & 15))) & 3u) ^ (rc ? 3u : 0u);
+        float w = cm.u.weight;
+        if (kQ != 0) {
+            const int qi = rc ? maxLen - 1 - pos : pos;
+            const u32* qr = L.qrows + c * qpitch;
+            float qw;
+            if (kQ == 8) {
+                qw = L.qw[(qr[qi >> 2] >> (8 * (qi & 3))) & 0xFFu];
+            } else if (kQ == 2) {
+                const u32 v = (qr[qi >> 4] >> (30 - 2 * (qi & 15))) & 3u;
+                qw = v == 0u ? q4[0] : v == 1u ? q4[1] : v == 2u ? q4[2] : q4[3];
+            } else {
+                qw = ((qr[qi >> 5] >> (31 - (qi & 31))) & 1u) ? q4[1] : q4[0];
+            }
+            w = cm.u.weight * qw;
+        }
+        msa_add(s, in ? b : 4u, w);
+    }
+}
+
 __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& dims, const CandMeta* meta, int m,
                          int la, int margin, int* sMin, int* sMax) {
     if (threadIdx.x == 0) {
@@ -201,33 +261,30 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
     const int rowsQ = qpitch ? int(blockDim.x) / qpitch : 0;
     const int r0Q = qpitch ? tx / qpitch : 0, wQ = tx - r0Q * qpitch;
     const int wb = max(0, msa.ab - margin), we = min(cols, ae + margin);
+    // quality width of the candidate rows, and the weights of the bin
+    // characters of 2-bit ('$', '(', '1', '9') and 1-bit ('(', '5') rows
+    const int qmode = useQ ? rs.qbits : 0;
+    float q4[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    if (qmode == 2) {
+        q4[0] = L.qw['$'];
+        q4[1] = L.qw['('];
+        q4[2] = L.qw['1'];
+        q4[3] = L.qw['9'];
+    } else if (qmode == 1) {
+        q4[0] = L.qw['('];
+        q4[1] = L.qw['5'];
+    }
     for (int cb = wb; cb < we; cb += blockDim.x) {
         const int col = cb + int(threadIdx.x);
         const bool active = col < we;
-        int cA = 0, cC = 0, cG = 0, cT = 0;
-        float wA = 0.f, wC = 0.f, wG = 0.f, wT = 0.f;
-        // conditional adds of +0.0f leave the other sums bit-identical
-        // (no sum is ever -0.0f)
-#define CARE_MSA_ADD(b, w)                        \
-    do {                                          \
-        const u32 b_ = (b);                       \
-        const float w_ = (w);                     \
-        cA += b_ == 0u;                           \
-        cC += b_ == 1u;                           \
-        cG += b_ == 2u;                           \
-        cT += b_ == 3u;                           \
-        wA += b_ == 0u ? w_ : 0.0f;               \
-        wC += b_ == 1u ? w_ : 0.0f;               \
-        wG += b_ == 2u ? w_ : 0.0f;               \
-        wT += b_ == 3u ? w_ : 0.0f;               \
-    } while (0)
+        MsaSums sums = {0, 0, 0, 0, 0.f, 0.f, 0.f, 0.f};
         u32 anchorBase = 0;
         const bool inAnchor = active && col >= msa.ab && col < ae;
         if (inAnchor) {
             const int i = col - msa.ab;
             anchorBase = base_at(L.aw, i);
             const float w = useQ ? L.qw[(unsigned char)L.anchorQ[i]] : 1.0f;
-            CARE_MSA_ADD(anchorBase, 1.0f * w);
+            msa_add(sums, anchorBase, 1.0f * w);
         }
         for (int c0 = 0; c0 < m; c0 += dims.chunk) {
             const int nc = min(dims.chunk, m - c0);
@@ -244,27 +301,15 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
                     L.qrows[c * qpitch + wQ] = rs.quals[u64(meta[c0 + c].id) * qpitch + wQ];
             __syncthreads();
             if (!active) continue;
-            // branch-free body (base 4 adds nothing), so the loop can be
-            // unrolled and its LDS reads overlapped
-#pragma unroll 4
-            for (int c = 0; c < nc; c++) {
-                const CandMeta cm = L.mst[c];
-                int pos = col - msa.ab - int(cm.shift);
-                const bool in = pos >= 0 && pos < int(cm.len);
-                pos = in ? pos : 0;
-                const bool rc = cm.flag == 2;
-                const u32 b = oriented_base(L.rows + c * pitch, cm.len, pos, rc);
-                float w = cm.u.weight;
-                if (useQ) {
-                    const int qi = rc ? rs.maxLen - 1 - pos : pos;
-                    const char q = quality_char(L.qrows + c * qpitch, qi, rs.qbits);
-                    w = cm.u.weight * L.qw[(unsigned char)q];
-                }
-                CARE_MSA_ADD(in ? b : 4u, w);
-            }
+            const int colRel = col - msa.ab;
+            if (qmode == 0) msa_add_staged<0>(L, nc, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            else if (qmode == 2) msa_add_staged<2>(L, nc, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            else if (qmode == 1) msa_add_staged<1>(L, nc, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            else msa_add_staged<8>(L, nc, colRel, pitch, qpitch, rs.maxLen, q4, sums);
         }
-#undef CARE_MSA_ADD
         if (!active) continue;
+        const int cA = sums.cA, cC = sums.cC, cG = sums.cG, cT = sums.cT;
+        const float wA = sums.wA, wC = sums.wC, wG = sums.wG, wT = sums.wT;
         u8 cons = 0;
         float cw = wA;
         if (wC > cw) { cons = 1; cw = wC; }
