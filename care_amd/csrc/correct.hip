@@ -300,12 +300,16 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
                 for (int c = r0Q; c < nc; c += rowsQ)
                     L.qrows[c * qpitch + wQ] = rs.quals[u64(meta[c0 + c].id) * qpitch + wQ];
             __syncthreads();
-            if (!active) continue;
+            // a wave-uniform test: lanes past the window run the loop with
+            // the others (harmlessly, their sums are dropped), which keeps
+            // its counter scalar
+            if (!__any(active)) continue;
             const int colRel = col - msa.ab;
-            if (qmode == 0) msa_add_staged<0>(L, nc, colRel, pitch, qpitch, rs.maxLen, q4, sums);
-            else if (qmode == 2) msa_add_staged<2>(L, nc, colRel, pitch, qpitch, rs.maxLen, q4, sums);
-            else if (qmode == 1) msa_add_staged<1>(L, nc, colRel, pitch, qpitch, rs.maxLen, q4, sums);
-            else msa_add_staged<8>(L, nc, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            const int ncU = __builtin_amdgcn_readfirstlane(nc);  // uniform: a scalar loop counter
+            if (qmode == 0) msa_add_staged<0>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            else if (qmode == 2) msa_add_staged<2>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            else if (qmode == 1) msa_add_staged<1>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            else msa_add_staged<8>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
         }
         if (!active) continue;
         const int cA = sums.cA, cC = sums.cC, cG = sums.cG, cT = sums.cT;
