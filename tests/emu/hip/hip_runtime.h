@@ -477,6 +477,9 @@ inline int __builtin_amdgcn_mov_dpp(int v, int ctrl, int, int, bool) {
     return emu::exchange(v, src);
 }
 inline int __builtin_amdgcn_readlane(int v, int lane) { return emu::exchange(v, lane & 63); }
+// every lane of an emulated wave reaches the call (no exec mask), so the
+// first active lane is lane 0
+inline int __builtin_amdgcn_readfirstlane(int v) { return emu::exchange(v, 0); }
 template <class T>
 inline T __shfl_up(T v, unsigned delta, int width = 64) {
     (void)width;
