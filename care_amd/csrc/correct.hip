@@ -920,9 +920,18 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         // candidate ids, lengths, 2-bit rows and pair flags of a chunk are
         // gathered into LDS by the whole block, so their global loads (and
         // the binary searches in the mate's candidate list) run in parallel
+        // the mate's candidate list is searched in LDS when it fits the
+        // alignment keys and survivor slices, which are free until this
+        // chunk's alignment starts (not in the gated modes, where its
+        // registers would cost a wavefront per SIMD)
+        const int mateCap = kGated ? -1 : 2 * dims.chunk + 2 * kSurvivors * dims.waves;
         for (int c0 = 0; c0 < n; c0 += dims.chunk) {
             const int nc = min(dims.chunk, n - c0);
-            __syncthreads();  // the previous chunk's rows are no longer read
+            __syncthreads();  // the previous chunk's rows and keys are no longer read
+            if (mateIds && mateN <= mateCap) {
+                for (int i = tid; i < mateN; i += blockDim.x) L.keys[i] = mateIds[i];
+                __syncthreads();
+            }
             for (int i = tid; i < nc * rs.pitch; i += blockDim.x) {
                 const int c = i / rs.pitch, w = i - c * rs.pitch;
                 const u32 cid = ids[c0 + c];
@@ -937,13 +946,22 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
                     if (mateIds) {
                         const u32 want = cid ^ 1u;
                         CARE_CHECK(mateN >= 0 && mateN <= P.maxCand);
-                    int lo2 = 0, hi2 = mateN;
-                        while (lo2 < hi2) {
-                            const int mid = (lo2 + hi2) >> 1;
-                            if (mateIds[mid] < want) lo2 = mid + 1;
-                            else hi2 = mid;
+                        int lo2 = 0, hi2 = mateN;
+                        if (mateN <= mateCap) {
+                            while (lo2 < hi2) {
+                                const int mid = (lo2 + hi2) >> 1;
+                                if (L.keys[mid] < want) lo2 = mid + 1;
+                                else hi2 = mid;
+                            }
+                            paired = (lo2 < mateN && L.keys[lo2] == want) ? 1 : 0;
+                        } else {
+                            while (lo2 < hi2) {
+                                const int mid = (lo2 + hi2) >> 1;
+                                if (mateIds[mid] < want) lo2 = mid + 1;
+                                else hi2 = mid;
+                            }
+                            paired = (lo2 < mateN && mateIds[lo2] == want) ? 1 : 0;
                         }
-                        paired = (lo2 < mateN && mateIds[lo2] == want) ? 1 : 0;
                     }
                     L.mst[c].paired = paired;
                 }
