@@ -184,9 +184,11 @@ __device__ __forceinline__ void msa_add(MsaSums& s, u32 b, float w) {
 // (relative to the anchor's first base).  kQ is the quality width (0: no
 // qualities, else 1, 2 or 8 bits per base); q4 holds the weights of the
 // 1- and 2-bit bins.  The body has no branch: the orientation selects
-// indices, not code paths.  Left rolled: unrolled copies hold more loads in
+// indices, not code paths.  kSkipMiss: a wavefront skips the candidates
+// that miss all of its columns (faster for the plain kernel, slower for
+// the gated one, whose candidates overlap the anchor more).  Left rolled: unrolled copies hold more loads in
 // flight and raise the registers of the calling kernels.
-template <int kQ>
+template <int kQ, bool kSkipMiss>
 __device__ __forceinline__ void msa_add_staged(const Lds& L, int nc, int colRel, int pitch, int qpitch, int maxLen,
                                                const float (&q4)[4], MsaSums& s) {
 #pragma unroll 1
@@ -195,6 +197,7 @@ __device__ __forceinline__ void msa_add_staged(const Lds& L, int nc, int colRel,
         const int len = cm.len;
         const int p0 = colRel - int(cm.shift);
         const bool in = u32(p0) < u32(len);
+        if (kSkipMiss && !__any(in)) continue;  // the candidate misses every column of this wavefront
         const int pos = in ? p0 : 0;
         const bool rc = cm.flag == 2;
         // reverse complement: base len-1-pos of the stored read, complemented
@@ -220,6 +223,7 @@ __device__ __forceinline__ void msa_add_staged(const Lds& L, int nc, int colRel,
     }
 }
 
+template <bool kSkipMiss>
 __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& dims, const CandMeta* meta, int m,
                          int la, int margin, int* sMin, int* sMax) {
     if (threadIdx.x == 0) {
@@ -306,10 +310,10 @@ __device__ Msa build_msa(const CorrectParams& P, const Lds& L, const LdsDims& di
             if (!__any(active)) continue;
             const int colRel = col - msa.ab;
             const int ncU = __builtin_amdgcn_readfirstlane(nc);  // uniform: a scalar loop counter
-            if (qmode == 0) msa_add_staged<0>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
-            else if (qmode == 2) msa_add_staged<2>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
-            else if (qmode == 1) msa_add_staged<1>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
-            else msa_add_staged<8>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            if (qmode == 0) msa_add_staged<0, kSkipMiss>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            else if (qmode == 2) msa_add_staged<2, kSkipMiss>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            else if (qmode == 1) msa_add_staged<1, kSkipMiss>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
+            else msa_add_staged<8, kSkipMiss>(L, ncU, colRel, pitch, qpitch, rs.maxLen, q4, sums);
         }
         if (!active) continue;
         const int cA = sums.cA, cC = sums.cC, cG = sums.cG, cT = sums.cT;
@@ -1122,7 +1126,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
         // MSA columns read beyond the anchor: candidate correction reaches
         // new_columns past it, gate features two columns past a corrected read
         const int margin = P.correctCandidates ? P.newColumns + 2 : 2;
-        Msa msa = build_msa(P, L, dims, cur, m, la, margin, &sMin, &sMax);
+        Msa msa = build_msa<!kGated>(P, L, dims, cur, m, la, margin, &sMin, &sMax);
         const int maxIter = kRefine ? P.maxRefinements : 0;  // the gated mode skips refinement
         for (int iter = 0; iter < maxIter && m > 0; iter++) {
             // src/msa.cpp:532-734
@@ -1200,7 +1204,7 @@ __global__ void __launch_bounds__(256) k_correct(CorrectParams P, LdsDims dims) 
             cur = other;
             other = t;
             m = m2;
-            msa = build_msa(P, L, dims, cur, m, la, margin, &sMin, &sMax);
+            msa = build_msa<!kGated>(P, L, dims, cur, m, la, margin, &sMin, &sMax);
         }
 
         // ---- phase D: anchor correction (src/msa.cpp:304-453, include/corrector.hpp:1423-1472) ----
