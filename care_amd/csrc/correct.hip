@@ -185,8 +185,8 @@ __device__ __forceinline__ void msa_add(MsaSums& s, u32 b, float w) {
 // qualities, else 1, 2 or 8 bits per base); q4 holds the weights of the
 // 1- and 2-bit bins.  The body has no branch: the orientation selects
 // indices, not code paths.  kSkipMiss: a wavefront skips the candidates
-// that miss all of its columns (faster for the plain kernel, slower for
-// the gated one, whose candidates overlap the anchor more).  Left rolled: unrolled copies hold more loads in
+// that miss all of its columns (measured faster in the plain kernel and
+// slower in the gated one).  Left rolled: unrolled copies hold more loads in
 // flight and raise the registers of the calling kernels.
 template <int kQ, bool kSkipMiss>
 __device__ __forceinline__ void msa_add_staged(const Lds& L, int nc, int colRel, int pitch, int qpitch, int maxLen,
